@@ -1,0 +1,122 @@
+"""getrates throughput on one MI355X: FP64 HIP kernel vs the host-core C/OpenMP oracle.
+
+One step = one getrates call over ng grid points of the named mechanism (inputs resident
+in HBM, species-major layout).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def cpu_baseline(mech_name, mech, target_s=10.0, sample_ng=1 << 14):
+    """C/OpenMP oracle on the host cores over a bounded sample of the same states."""
+    import numpy as np
+    from chemgen_amd.synth import make_states
+    from oracle.build import COracle
+    co = COracle(mech_name)
+    P, T, Y = make_states(mech.n_species, sample_ng, seed=1)
+    co.getrates(P, T, Y)  # warm
+    t0 = time.perf_counter()
+    co.getrates(P, T, Y)
+    one = time.perf_counter() - t0
+    reps = max(1, min(200, int(target_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        co.getrates(P, T, Y)
+    dt = time.perf_counter() - t0
+    return {"value": sample_ng * reps / dt, "unit": "points/s", "cores": co.threads,
+            "kind": "port",
+            "sample": f"{reps} x {sample_ng} points of {mech_name} (seed 1), C/OpenMP restatement of "
+                      f"the reference's getrates_i, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mech", default="synth_gri53")
+    ap.add_argument("--ng", type=int, default=1 << 20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the getrates path is per-point independent and "
+              "this build runs on a single MI355X; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from chemgen_amd.kernels import load_bundled
+    from chemgen_amd.roofline import PEAK_FP64_VECTOR, work_per_point
+    from chemgen_amd.runtime import DeviceMechanism
+    from chemgen_amd.synth import make_states
+
+    mech = load_bundled(args.mech)
+    dm = DeviceMechanism(mech, build=False)
+    ng = args.ng
+    P, T, Y = make_states(mech.n_species, ng, seed=0)
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
+    Wd = torch.empty_like(Yd)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        dm.getrates_(Pd, Td, Yd, Wd, stream)
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for s in range(args.steps):
+        dm.getrates_(Pd, Td, Yd, Wd, stream)
+        evs[s + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [evs[s].elapsed_time(evs[s + 1]) for s in range(args.steps)]
+    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    ms_per_step = wall / args.steps * 1e3
+
+    # parity of the benchmarked output on a sample, against the C oracle
+    from oracle.build import COracle
+    from oracle.getrates_np import getrates as np_oracle
+    from oracle.metrics import parity_stats
+    nchk = 2048
+    Wc = Wd[:, :nchk].cpu().numpy()
+    ref, qf, qb = np_oracle(mech, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
+    par = parity_stats(mech, Wc, ref, qf, qb)
+
+    w = work_per_point(mech)
+    achieved = w["flops"] * ng / avg_launch_s
+    out = {
+        "metric": "getrates points/s (FP64 HIP, GRI-sized synthetic mechanism)",
+        "value": ng / (ms_per_step / 1e3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.mech} ({mech.n_species} species, {mech.n_reactions} reactions), "
+                               f"ng={ng}, T U(300,2500) K, P 1-40 atm log-uniform, Y Dirichlet(0.5), seed 0",
+                   "mechanism": args.mech, "ng": ng, "kernel_variant": dm.variant.tag},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": PEAK_FP64_VECTOR / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VECTOR, "traffic": None,
+                     "flops_per_point": w["flops"], "bytes_per_point": w["bytes"],
+                     "kernel_avg_ms": avg_launch_s * 1e3},
+        "parity": {"sample_points": nchk, **par},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

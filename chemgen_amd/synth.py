@@ -1,0 +1,165 @@
+"""Seeded synthetic mechanisms and grid-point states for benchmarks and tests.
+
+A synthetic mechanism keeps real species (composition, NASA-7 thermo perturbed by up to
++/-5% per coefficient) and draws element-balanced reactions among them:
+log10 A ~ U(6, 16), beta ~ U(-2, 3), Ea ~ U(0, 60) kcal/mol, a fixed number of Troe /
+Lindemann / SRI falloff and third-body reactions with random efficiencies in
+[0.5, 12].  Most bimolecular reactions pair an arbitrary species with one of the small
+radicals/molecules of the set, the shape real combustion mechanisms have, and every
+reaction is written in its exergonic direction at 1000 K, as mechanisms are.  The result is
+written as Chemkin text and parsed back, so the mechanism used downstream is exactly
+what the Chemkin reader produces from the file.
+
+States follow one recipe everywhere: ``rng = default_rng(seed)``; ``T ~ U(Tlo, Thi)``;
+``P = 1 atm * exp(U(ln Plo, ln Phi))`` in dyn/cm^2; ``Y ~ Dirichlet(0.5)`` renormalised.
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .mechanism import Mechanism, Nasa7, Reaction, Species, parse_chemkin, write_chemkin
+
+KCAL = 1000.0
+
+
+def make_states(n_species: int, ng: int, seed: int = 0, T_range=(300.0, 2500.0),
+                P_atm=(1.0, 40.0)):
+    """Seeded states: returns P[ng], T[ng], Y[Ns, ng] (species-major), all float64."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(T_range[0], T_range[1], ng)
+    P = 1013250.0 * np.exp(rng.uniform(math.log(P_atm[0]), math.log(P_atm[1]), ng))
+    Y = rng.dirichlet(0.5 * np.ones(n_species), ng)
+    Y /= Y.sum(axis=1, keepdims=True)
+    return P, T, np.ascontiguousarray(Y.T)
+
+
+def _perturb_thermo(th: Nasa7, rng, frac: float) -> Nasa7:
+    f = lambda c: tuple(float(x * (1.0 + rng.uniform(-frac, frac))) for x in c)
+    return Nasa7(th.t_min, th.t_mid, th.t_max, f(th.low), f(th.high))
+
+
+def synthetic_mechanism(base_species: Sequence[Species], elements: Sequence[str], n_reactions: int,
+                        n_troe: int, n_lindemann: int, n_sri: int, n_third_body: int,
+                        seed: int, name: str, frac_irreversible: float = 0.05,
+                        thermo_perturb: float = 0.05, n_small: int = 12) -> Mechanism:
+    rng = np.random.default_rng(seed)
+    els = list(elements)
+    species = [Species(s.name, dict(s.composition), _perturb_thermo(s.thermo, rng, thermo_perturb))
+               for s in base_species]
+    ns = len(species)
+    comp = np.array([[s.composition.get(e, 0.0) for e in els] for s in species], dtype=np.int64)
+    key = lambda v: tuple(int(x) for x in v)
+    by_comp: Dict[tuple, List[int]] = {}
+    for k in range(ns):
+        by_comp.setdefault(key(comp[k]), []).append(k)
+    pair_sum: Dict[tuple, List[Tuple[int, int]]] = {}
+    for i in range(ns):
+        for j in range(i, ns):
+            pair_sum.setdefault(key(comp[i] + comp[j]), []).append((i, j))
+    natoms = comp.sum(axis=1)
+    small = [int(k) for k in np.argsort(natoms, kind="stable")[:n_small] if natoms[k] > 0]
+    splittable = [k for k in range(ns) if len(pair_sum.get(key(comp[k]), [])) > 0]
+    names = [s.name for s in species]
+
+    def arrh():
+        return (10.0 ** rng.uniform(6.0, 16.0), float(rng.uniform(-2.0, 3.0)),
+                float(rng.uniform(0.0, 60.0)) * KCAL)
+
+    def effs():
+        n = int(rng.integers(3, 9))
+        pick = rng.choice(ns, size=min(n, ns), replace=False)
+        return {names[int(k)]: float(rng.uniform(0.5, 12.0)) - 1.0 for k in pick}
+
+    def st(idx):
+        d: Dict[str, float] = {}
+        for k in idx:
+            d[names[k]] = d.get(names[k], 0.0) + 1.0
+        return d
+
+    def assoc():
+        """A + B <=> AB (randomly oriented)."""
+        for _ in range(10000):
+            k = splittable[int(rng.integers(len(splittable)))]
+            pairs = pair_sum[key(comp[k])]
+            i, j = pairs[int(rng.integers(len(pairs)))]
+            if k in (i, j):
+                continue
+            if rng.uniform() < 0.5:
+                return [i, j], [k]
+            return [k], [i, j]
+        raise RuntimeError("no association reaction found")
+
+    def bimol():
+        for _ in range(100000):
+            i = int(rng.integers(ns))
+            j = small[int(rng.integers(len(small)))] if rng.uniform() < 0.7 else int(rng.integers(ns))
+            cands = [p for p in pair_sum.get(key(comp[i] + comp[j]), []) if sorted(p) != sorted((i, j))]
+            if not cands:
+                continue
+            p = cands[int(rng.integers(len(cands)))]
+            return [i, j], list(p)
+        raise RuntimeError("no bimolecular reaction found")
+
+    def smh1000(k):
+        a = species[k].thermo.high if 1000.0 >= species[k].thermo.t_mid else species[k].thermo.low
+        T = 1000.0
+        return (a[6] - a[0] + a[0] * math.log(T) + a[1] / 2 * T + a[2] / 6 * T ** 2
+                + a[3] / 12 * T ** 3 + a[4] / 20 * T ** 4 - a[5] / T)
+
+    def orient(r_idx, p_idx):
+        """Write the reaction in its exergonic direction at 1000 K, as mechanisms are."""
+        d = sum(smh1000(k) for k in p_idx) - sum(smh1000(k) for k in r_idx)
+        return (r_idx, p_idx) if d >= 0.0 else (p_idx, r_idx)
+
+    kinds = (["troe"] * n_troe + ["lindemann"] * n_lindemann + ["sri"] * n_sri
+             + ["third_body"] * n_third_body)
+    kinds += ["standard"] * (n_reactions - len(kinds))
+    order = rng.permutation(len(kinds))
+    reactions: List[Reaction] = []
+    for o in order:
+        kind = kinds[int(o)]
+        rev = bool(rng.uniform() >= frac_irreversible)
+        if kind == "standard":
+            r_idx, p_idx = orient(*(bimol() if rng.uniform() < 0.85 else assoc()))
+            r = Reaction("", st(r_idx), st(p_idx), "standard", rev, arrh())
+        elif kind == "third_body":
+            r_idx, p_idx = orient(*assoc())
+            r = Reaction("", st(r_idx), st(p_idx), "third_body", rev, arrh())
+            r.third_body = effs()
+        else:
+            r_idx, p_idx = orient(*assoc())
+            A, b, Ea = arrh()
+            low = (A * 10.0 ** rng.uniform(3.0, 8.0), float(rng.uniform(-3.0, 0.0)),
+                   Ea * float(rng.uniform(0.0, 1.0)))
+            r = Reaction("", st(r_idx), st(p_idx), kind, rev, (A, b, Ea), low=low)
+            r.third_body = effs()
+            if kind == "troe":
+                tp = (float(rng.uniform(0.2, 0.95)), 10.0 ** rng.uniform(1.5, 3.0),
+                      10.0 ** rng.uniform(2.5, 4.0))
+                if rng.uniform() < 0.5:
+                    tp = tp + (10.0 ** rng.uniform(3.0, 4.0),)
+                r.troe = tp
+            elif kind == "sri":
+                r.sri = (float(rng.uniform(0.1, 1.0)), float(rng.uniform(50.0, 500.0)),
+                         10.0 ** rng.uniform(2.5, 3.5), float(rng.uniform(0.5, 2.0)),
+                         float(rng.uniform(-0.5, 0.5)))
+        reactions.append(r)
+    return Mechanism(name, els, species, reactions)
+
+
+def write_and_parse(mech: Mechanism, path: str) -> Mechanism:
+    write_chemkin(mech, path)
+    return parse_chemkin(path, name=mech.name)
+
+
+# sizes of the benchmark mechanisms: (species source, reactions, troe, lindemann, sri,
+# third-body, seed)
+SPECS = {
+    "synth_gri53": dict(n_reactions=325, n_troe=25, n_lindemann=4, n_sri=0, n_third_body=40, seed=53),
+    "synth_nhept88": dict(n_reactions=387, n_troe=33, n_lindemann=6, n_sri=0, n_third_body=20, seed=88),
+    "synth_jetsurf348": dict(n_reactions=2163, n_troe=150, n_lindemann=0, n_sri=23, n_third_body=324,
+                             seed=348),
+}

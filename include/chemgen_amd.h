@@ -1,0 +1,93 @@
+/* chemgen_amd: C ABI of the MI355X (gfx950) getrates library.
+ *
+ * The reference exposes its HIP getrates through
+ *   extern "C" void getrates_hip_wrapper_vN(double* P, double* Temp, int ng, double* Ysp,
+ *                                           double* wdot_rr, double* log_eg, double* log_c);
+ *   (chemgen/templates/reactions_hip/hip_v1_h.j2:7-13, hip_v2_h.j2:7-13, hip_v3_h.j2:7-13,
+ *    hip_v4_h.j2:26-32)
+ * plus a per-mechanism constant upload
+ *   extern "C" void copyConstantsToDevice(...)      (chemgen/constants.py:119-240)
+ * and calls them from Fortran through chemgen_m::init_chemgen / getrates_gpu
+ * (chemgen/templates/chemgen_mod.j2:26-166).
+ *
+ * This header replaces that pair with a handle-based API: one mechanism per handle
+ * (code object + gpu_coef blob) instead of one per process, explicit HIP streams,
+ * error codes instead of void, tails of any length, and a leading dimension for the
+ * species-major (SoA) arrays.  Every pointer passed to a compute call is a DEVICE
+ * pointer; Y and wdot are laid out as Y[k*ld + i] (species k, grid point i), which is
+ * the reference's Fortran Y(ng,Ns) column-major layout (device_functions_h.j2:712)
+ * with ld = ng.  Calls enqueue work on `stream` (a hipStream_t; NULL = default stream)
+ * and do not synchronise; the caller synchronises the stream before reading wdot.
+ * Calls on one handle from several host threads must be serialised by the caller.
+ *
+ * Return value of every function: 0 on success, a negative CG_E* code otherwise;
+ * cg_last_error() gives the message of the calling thread's last failure.
+ */
+#ifndef CHEMGEN_AMD_H
+#define CHEMGEN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CG_OK 0
+#define CG_EINVAL (-1)   /* bad argument (null pointer, negative size, ld < ng, ...) */
+#define CG_EHIP (-2)     /* a HIP runtime call failed */
+#define CG_ELOAD (-3)    /* code object missing / malformed / wrong mechanism */
+#define CG_ENOMEM (-4)   /* device allocation failed */
+#define CG_ERANGE (-5)   /* size beyond what the entry point supports */
+
+typedef struct cg_mech* cg_mech_t;
+
+/* Message of this thread's last failure ("" if none). */
+const char* cg_last_error(void);
+
+/* Load a generated getrates code object and its gpu_coef blob on the current device.
+ * Replaces copyConstantsToDevice (chemgen/constants.py:119-240): the coefficients are
+ * copied into device memory owned by the handle.  `expect_hash` must equal the
+ * structure hash compiled into the code object (0 skips the check). */
+int cg_mech_load(const char* code_object_path, const double* coef_host, int64_t n_coef,
+                 uint64_t expect_hash, cg_mech_t* out);
+
+/* Replace the coefficient values of a loaded mechanism (same structure, e.g. perturbed
+ * Arrhenius or thermo data).  n_coef must match. */
+int cg_mech_set_coef(cg_mech_t h, const double* coef_host, int64_t n_coef);
+
+/* Sizes compiled into the code object. */
+int cg_mech_info(cg_mech_t h, int32_t* n_species, int32_t* n_reactions, int32_t* block,
+                 uint64_t* structure_hash);
+
+int cg_mech_free(cg_mech_t h);
+
+/* wdot = getrates(P, T, Y) for ng grid points: P [dyn/cm^2], T [K], Y mass fractions,
+ * wdot [mol/cm^3/s].  Replaces getrates_hip_wrapper_vN (hip_v1_h.j2:7-13): no scratch
+ * arguments (log_eg/log_c are not needed), no dropped tail when ng is not a multiple
+ * of the block, asynchronous on `stream`.  ld >= ng. */
+int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_t ng, int64_t ld,
+                const double* Y, double* wdot, void* stream);
+
+/* Ragged batch in one launch.  Patch b covers points [offsets[b], offsets[b+1]) of the
+ * packed arrays (offsets is a HOST array of n_patches+1 entries, offsets[0] = 0,
+ * non-decreasing).  The tile list is built on the host and uploaded into a buffer the
+ * handle owns (grown on demand), then one persistent kernel walks it; the launch is
+ * capturable into a hipGraph once the buffer is large enough.  `grid` = number of
+ * persistent workgroups (0 = 8 per CU).  Reference: no counterpart (the reference
+ * launches one kernel per array, hip_v1_cpp.j2:339-372). */
+int cg_getrates_batched(cg_mech_t h, const double* P, const double* T, const double* Y,
+                        double* wdot, int64_t ld, const int64_t* offsets, int32_t n_patches,
+                        int32_t grid, void* stream);
+
+/* Prepare the tile list of a ragged batch without launching (so that a following
+ * cg_getrates_batched_prepared can be captured into a hipGraph). */
+int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n_patches);
+int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const double* T,
+                                 const double* Y, double* wdot, int64_t ld, int32_t grid,
+                                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHEMGEN_AMD_H */

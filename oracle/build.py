@@ -1,0 +1,73 @@
+"""Build the C oracle (oracle/cgen.py output) for the bundled mechanisms -- TEST INFRASTRUCTURE.
+
+Each mechanism becomes oracle/_build/oracle_<name>.so, compiled with
+``gcc -O3 -march=x86-64-v3 -fopenmp`` (AVX2/FMA, no fast-math, so libm's exp/pow/log keep
+their accuracy).  x86-64-v3 rather than -march=native: the library is built here and
+also runs on the GPU host.
+"""
+import ctypes
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "_build")
+MECHS = ("h2_burke", "h2_burke2012", "gri30", "synth_gri53", "synth_nhept88", "synth_jetsurf348")
+CFLAGS = ["-O3", "-march=x86-64-v3", "-fopenmp", "-fno-math-errno", "-shared", "-fPIC"]
+
+
+def so_path(name: str) -> str:
+    return os.path.join(BUILD, f"oracle_{name}.so")
+
+
+def build_one(mech, name: str = None) -> str:
+    from oracle.cgen import emit_c
+    name = name or mech.name
+    os.makedirs(BUILD, exist_ok=True)
+    src = os.path.join(BUILD, f"oracle_{name}.c")
+    with open(src, "w") as fh:
+        fh.write(emit_c(mech))
+    out = so_path(name)
+    res = subprocess.run(["gcc", *CFLAGS, src, "-o", out + ".tmp", "-lm"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"gcc failed for {src}:\n{res.stderr[-3000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(names=MECHS, jobs=6):
+    sys.path.insert(0, ROOT)
+    from chemgen_amd.kernels import load_bundled
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda n: build_one(load_bundled(n), n), names))
+
+
+class COracle:
+    """ctypes handle on one built C oracle."""
+
+    def __init__(self, name: str):
+        path = so_path(name)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path}: run oracle.build.build_all() first")
+        self.lib = ctypes.CDLL(path)
+        self.lib.oracle_getrates.argtypes = [ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 4
+        self.lib.oracle_getrates.restype = None
+        self.ns = self.lib.oracle_n_species()
+        self.threads = self.lib.oracle_threads()
+
+    def getrates(self, P, T, Y):
+        import numpy as np
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        assert Y.shape[0] == self.ns
+        ng = P.shape[0]
+        W = np.empty_like(Y)
+        self.lib.oracle_getrates(ng, Y.shape[1], P.ctypes.data, T.ctypes.data, Y.ctypes.data, W.ctypes.data)
+        return W
+
+
+if __name__ == "__main__":
+    build_all()
