@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
+                    help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the getrates path is per-point independent and "
@@ -52,13 +54,23 @@ def main():
 
     import numpy as np
     import torch
+    if args.mode == "copy":
+        n = 1 << 27  # 2^27 doubles = 1 GiB read + 1 GiB written per copy
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read_per_launch": 8 * n, "bytes_written_per_launch": 8 * n}))
+        return
     from chemgen_amd.kernels import load_bundled
     from chemgen_amd.roofline import PEAK_FP64_VECTOR, work_per_point
     from chemgen_amd.runtime import DeviceMechanism
     from chemgen_amd.synth import make_states
 
+    from chemgen_amd.autotune import best_variant
     mech = load_bundled(args.mech)
-    dm = DeviceMechanism(mech, build=False)
+    dm = DeviceMechanism(mech, best_variant(mech), build=False)
     ng = args.ng
     P, T, Y = make_states(mech.n_species, ng, seed=0)
     dev = torch.device("cuda:0")

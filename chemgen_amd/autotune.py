@@ -1,0 +1,112 @@
+"""Launch-shape / code-shape autotuner for the generated getrates kernels.
+
+The reference picks its HIP launch shape by hand through ``--parallel-level``, ``--veclen``
+and ``--nreact-per-block`` (generate_chemistry.py:20-26) and a sweep of mini-app builds
+(compile_mini_apps.sh, compare_mini_apps_write_csv.sh).  Here each candidate is a
+:class:`Variant` (points per block, register budget via waves/EU, exp/log
+implementation, reverse-rate formulation, polynomial scheme, EG placement), compiled
+ahead of time; the tuner times every candidate on the GPU with HIP events, checks that
+all candidates agree with the first one, and records the winner per mechanism structure
+in ``data/tuning.json`` so later loads pick it without re-measuring.
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+from typing import Dict, List, Optional
+
+from .codegen import Variant, structure_hash
+from .mechanism import Mechanism
+
+TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning.json")
+
+
+def candidate_variants(mech: Mechanism) -> List[Variant]:
+    """Candidates swept by default (all compiled by the build for the bundled mechanisms)."""
+    out = [Variant(64, 1, "lean", "eg", "horner", False),   # lane per point, state in registers
+           Variant(64, 2, "lean", "eg", "horner", True, 4)]  # lane per point, EG in LDS
+    for split, waves in ((4, 2), (4, 3), (6, 3), (8, 3), (8, 4)):   # reaction-split tiles
+        for group in (4, 8):
+            out.append(Variant(64, waves, "lean", "eg", "horner", False, group, split))
+    return out
+
+
+def _load_db() -> Dict:
+    if os.path.exists(TUNING_PATH):
+        with open(TUNING_PATH) as fh:
+            return json.load(fh)
+    return {}
+
+
+def best_variant(mech: Mechanism) -> Variant:
+    """The tuned variant for this mechanism structure, or the default shape."""
+    db = _load_db()
+    e = db.get(mech.name)
+    if e and e.get("structure_hash") == f"{structure_hash(mech):016x}":
+        return Variant(**e["best"])
+    return Variant()
+
+
+def tune(mech: Mechanism, ng: int = 1 << 20, variants: Optional[List[Variant]] = None,
+         reps: int = 10, seed: int = 0, record: bool = True, build: bool = False,
+         out_path: Optional[str] = None) -> Dict:
+    """Time each variant on cuda:0 over ng seeded states; returns (and records) the results."""
+    import numpy as np
+    import torch
+    from .runtime import DeviceMechanism
+    from .synth import make_states
+    variants = variants or candidate_variants(mech)
+    P, T, Y = make_states(mech.n_species, ng, seed=seed)
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
+    W = torch.empty_like(Yd)
+    ref = None
+    results = []
+    for v in variants:
+        print(f"[autotune] {mech.name} {v.tag}", flush=True)
+        dm = DeviceMechanism(mech, v, build=build)
+        for _ in range(2):
+            dm.getrates_(Pd, Td, Yd, W)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            dm.getrates_(Pd, Td, Yd, W)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        w = W[:, :4096].cpu().numpy()
+        if ref is None:
+            ref = w
+            agree = 0.0
+        else:
+            scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1e-300)
+            agree = float(np.max(np.abs(w - ref) / scale))
+        ok = bool(np.isfinite(w).all()) and agree <= 1e-10
+        results.append({"variant": dataclasses.asdict(v), "tag": v.tag, "ms": ms,
+                        "points_per_s": ng / (ms / 1e3), "agree": agree, "ok": ok})
+        print(json.dumps(results[-1]), flush=True)
+        dm.close()
+    good = [r for r in results if r["ok"]]
+    best = min(good, key=lambda r: r["ms"]) if good else None
+    entry = {"structure_hash": f"{structure_hash(mech):016x}", "ng": ng,
+             "best": best["variant"] if best else dataclasses.asdict(Variant()), "results": results}
+    if record and best:
+        db = _load_db()
+        db[mech.name] = entry
+        with open(out_path or TUNING_PATH, "w") as fh:
+            json.dump(db, fh, indent=1, sort_keys=True)
+    return entry
+
+
+if __name__ == "__main__":
+    import argparse
+    from .kernels import load_bundled
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mechs", nargs="*", default=["synth_gri53"])
+    ap.add_argument("--ng", type=int, default=1 << 20)
+    ap.add_argument("--out", default=None, help="where to write the tuning database")
+    a = ap.parse_args()
+    for name in a.mechs:
+        tune(load_bundled(name), ng=a.ng, out_path=a.out)

@@ -9,16 +9,19 @@ zero coefficients) into straight-line code, one lane per grid point, and keeps t
 
   [invmw[Ns] | smh[Ns][14] | per-reaction records]
 
-smh record per species: low range (a6-a0, a0, a1/2, a2/6, a3/12, a4/20, -a5), then the
-high range -- the transform of chemgen/write_constants.py:146-189 -- evaluated as
-``c0 + c1 lnT + T(c2 + T(c3 + T(c4 + T c5))) + c6/T``.
+The kernels work in base 2 (one exp2 per Arrhenius rate, no ln 2 reduction step), so the
+blob holds base-2 forms; L = log2(e):
+
+smh record per species: low range L*(a6-a0), a0, L*a1/2, L*a2/6, L*a3/12, L*a4/20, -L*a5,
+then the high range -- the transform of chemgen/write_constants.py:146-189 scaled by L --
+evaluated as ``c0 + c1 log2T + c2 T + c3 T^2 + c4 T^3 + c5 T^4 + c6/T = smh log2(e)``.
 
 Reaction records (kernel order = mechanism order):
-  standard/third-body: lnA, beta, Ea/Rc, then (eff-1) per listed third-body species
-  falloff: lnA_inf, beta_inf, Ea_inf/Rc, lnA_0, beta_0, Ea_0/Rc, then
-           Troe (1-a, 1/T3, a, 1/T1[, T2]) | SRI (a, b, 1/c, d, e) | Lindemann (none),
+  standard/third-body: log2 A, beta, L Ea/Rc, then (eff-1) per listed third-body species
+  falloff: the same three for the high- and the low-pressure limit, then
+           Troe (1-a, L/T3, a, L/T1[, L T2]) | SRI (a, L b, L/c, d, e) | Lindemann (none),
            then (eff-1) per listed species
-  plog: per pressure point (ln p, n_rates, lnA, beta, Ea/Rc ...)
+  plog: per pressure point (ln p, p, then A, beta, -Ea per rate) -- read by no kernel yet
 
 Numbers are taken after the reference's 16-significant-digit printing (``%.15e``,
 chemgen/templates/reactions/standard.j2:5, chemgen/write_constants.py:14-70).
@@ -41,6 +44,9 @@ R0 = 8.314510e+07
 RC = 1.9872155832
 PATM = 1013250.0
 LN10 = math.log(10.0)
+LOG2E = 1.0 / math.log(2.0)
+LOG2_10 = math.log2(10.0)
+LOG10_2 = math.log10(2.0)
 
 TEMPLATE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "templates")
 
@@ -59,10 +65,30 @@ class Variant:
     """
     block: int = 64
     waves_per_eu: int = 1
+    exp_impl: str = "lean"   # "lean": own exp/log (range reduction + polynomials); "ocml": libm
+    kr_mode: str = "eg"      # "eg": kb from per-species exp(smh) products; "exp": one exp per kb
+    poly: str = "horner"     # exp polynomial evaluation: "horner" or "estrin" (shorter chains)
+    g_lds: bool = False      # keep the per-species EG_k in LDS instead of registers
+    group: int = 4           # with LDS state: reactions between re-reads from LDS
+    split: int = 1           # wavefronts per 64-point tile sharing the species state in LDS
+                             # (each runs 1/split of the reactions); 1 = lane-per-point kernel
 
     @property
     def tag(self) -> str:
-        return f"b{self.block}w{self.waves_per_eu}"
+        if self.split > 1:
+            return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
+                    f"{self.poly[0]}g{self.group}")
+        return (f"b{self.block}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
+                f"{self.poly[0]}{'L%d' % self.group if self.g_lds else ''}")
+
+    @property
+    def threads(self) -> int:
+        return 64 * self.split if self.split > 1 else self.block
+
+    @property
+    def ppb(self) -> int:
+        """Grid points per workgroup."""
+        return 64 if self.split > 1 else self.block
 
 
 class _Coef:
@@ -76,17 +102,18 @@ class _Coef:
 
 
 def _lk(off: int, beta: float, ea: float) -> str:
-    """ln k = lnA + beta lnT - (Ea/Rc)/T with zero terms compiled out."""
-    e = f"cf[{off}]"
-    if beta != 0.0:
-        e = f"__builtin_fma(cf[{off + 1}], lnT, {e})"
+    """log2 k = log2 A + beta log2 T - (Ea/(Rc ln2))/T with zero terms compiled out; written
+    so that every operation takes at most one coefficient (one scalar operand per VALU op)."""
+    terms = None
     if ea != 0.0:
-        e = f"__builtin_fma(-cf[{off + 2}], invT, {e})"
-    return e
+        terms = f"(-cf[{off + 2}] * invT)"
+    if beta != 0.0:
+        terms = f"(cf[{off + 1}] * lg2T)" if terms is None else f"__builtin_fma(cf[{off + 1}], lg2T, {terms})"
+    return f"cf[{off}]" if terms is None else f"({terms} + cf[{off}])"
 
 
 def _arr_vals(A: float, beta: float, Ea: float) -> Tuple[float, float, float]:
-    return (math.log(r15(A)), float(beta), r15(Ea) / RC)
+    return (math.log2(r15(A)), float(beta), r15(Ea) / RC * LOG2E)
 
 
 def structure_key(mech: Mechanism) -> str:
@@ -122,10 +149,11 @@ def build_coef(mech: Mechanism):
     for s in mech.species:
         th = s.thermo
         for a in (th.low, th.high):
-            cb.add(r15(a[6] - a[0]), a[0] if a[0] == 0 else math.copysign(r15(abs(a[0])), a[0]),
-                   math.copysign(r15(abs(a[1] / 2.0)), a[1]), math.copysign(r15(abs(a[2] / 6.0)), a[2]),
-                   math.copysign(r15(abs(a[3] / 12.0)), a[3]), math.copysign(r15(abs(a[4] / 20.0)), a[4]),
-                   -math.copysign(r15(abs(a[5])), a[5]))
+            L = LOG2E
+            cb.add(L * r15(a[6] - a[0]), a[0] if a[0] == 0 else math.copysign(r15(abs(a[0])), a[0]),
+                   L * math.copysign(r15(abs(a[1] / 2.0)), a[1]), L * math.copysign(r15(abs(a[2] / 6.0)), a[2]),
+                   L * math.copysign(r15(abs(a[3] / 12.0)), a[3]), L * math.copysign(r15(abs(a[4] / 20.0)), a[4]),
+                   -L * math.copysign(r15(abs(a[5])), a[5]))
     roffs: List[int] = []
     for r in mech.reactions:
         if r.rtype in ("standard", "third_body"):
@@ -135,13 +163,13 @@ def build_coef(mech: Mechanism):
             cb.add(*_arr_vals(*r.low))
             if r.rtype == "troe":
                 a, T3, T1 = r.troe[0], r.troe[1], r.troe[2]
-                cb.add(r15(1.0 - a), (1.0 / r15(T3)) if T3 != 0.0 else float("inf"), r15(a),
-                       (1.0 / r15(T1)) if T1 != 0.0 else float("inf"))
+                cb.add(r15(1.0 - a), (LOG2E / r15(T3)) if T3 != 0.0 else float("inf"), r15(a),
+                       (LOG2E / r15(T1)) if T1 != 0.0 else float("inf"))
                 if len(r.troe) == 4:
-                    cb.add(r15(r.troe[3]))
+                    cb.add(LOG2E * r15(r.troe[3]))
             elif r.rtype == "sri":
                 sp = tuple(r.sri) + ((1.0, 0.0) if len(r.sri) == 3 else ())
-                cb.add(r15(sp[0]), r15(sp[1]), 1.0 / r15(sp[2]), r15(sp[3]), float(sp[4]))
+                cb.add(r15(sp[0]), LOG2E * r15(sp[1]), LOG2E / r15(sp[2]), r15(sp[3]), float(sp[4]))
         elif r.rtype == "plog":
             pts = sorted(set(c[0] for c in r.plog))
             off = len(cb.vals)
@@ -158,7 +186,23 @@ def build_coef(mech: Mechanism):
     return np.array(cb.vals, dtype=np.float64), off_smh, roffs
 
 
-def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int]) -> List[str]:
+def _pfac_pow(dnu: float) -> str:
+    """pfac^(-dnu) as a product (kb = kf K_c^-1, K_c = exp(dsmh) pfac^dnu)."""
+    if dnu == 0.0:
+        return ""
+    if float(dnu).is_integer() and abs(dnu) <= 3:
+        base = "pinv" if dnu > 0 else "pfac"
+        return " * " + " * ".join([base] * int(abs(dnu)))
+    return f" * cg_exp2({dlit(-dnu)} * lg2pfac)"
+
+
+def _kr_eg(r: Reaction, idx: Dict[str, int], kf: str) -> str:
+    num = stoich_product(power_expr(f"g{idx[s]}", n) for s, n in r.reactants.items())
+    den = stoich_product(power_expr(f"g{idx[s]}", n) for s, n in r.products.items())
+    return f"cg_div_fast({kf} * {num}, {den}){_pfac_pow(r.coeff_sum)}"
+
+
+def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp") -> List[str]:
     L = [f"  {{  // R{j}: {_eq(r)}"]
     cr = stoich_product(power_expr(f"c{idx[s]}", n) for s, n in r.reactants.items())
     cp = stoich_product(power_expr(f"c{idx[s]}", n) for s, n in r.products.items())
@@ -171,15 +215,18 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int]) -> List[s
     dnu = r.coeff_sum
     dg = " ".join(dg_terms)
     if dnu != 0.0:
-        dg += f" + {dlit(dnu)} * lnpfac"
+        dg += f" + {dlit(dnu)} * lg2pfac"
     if dg.startswith("+ "):
         dg = dg[2:]
     tb_off = None
     if r.rtype in ("standard", "third_body"):
         L.append(f"    const double lk = {_lk(off, r.arrh[1], r.arrh[2])};")
-        L.append("    const double kf = cg_exp(lk);")
+        L.append("    const double kf = cg_exp2_fin(lk);")
         if r.reversible:
-            L.append(f"    const double kr = cg_exp(lk - ({dg}));")
+            if kr_mode == "eg":
+                L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
+            else:
+                L.append(f"    const double kr = cg_exp2(lk - ({dg}));")
         tb_off = off + 3
     elif r.rtype in ("troe", "lindemann", "sri"):
         nx = {"troe": 4 + (1 if r.troe and len(r.troe) == 4 else 0), "lindemann": 0, "sri": 5}[r.rtype]
@@ -196,33 +243,36 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int]) -> List[s
     if r.rtype in ("troe", "lindemann", "sri"):
         L.append(f"    const double lki = {_lk(off, r.arrh[1], r.arrh[2])};")
         L.append(f"    const double lk0 = {_lk(off + 3, r.low[1], r.low[2])};")
-        L.append("    const double kinf = cg_exp(lki);")
-        L.append("    const double pr = M * cg_exp(lk0 - lki);")
+        L.append("    const double kinf = cg_exp2_fin(lki);")
+        L.append("    const double pr = M * cg_exp2_fin(lk0 - lki);")
         o = off + 6
         if r.rtype == "troe":
-            fc = f"cf[{o}] * cg_exp(-t * cf[{o + 1}]) + cf[{o + 2}] * cg_exp(-t * cf[{o + 3}])"
+            fc = f"cf[{o}] * cg_exp2(-t * cf[{o + 1}]) + cf[{o + 2}] * cg_exp2(-t * cf[{o + 3}])"
             if len(r.troe) == 4:
-                fc += f" + cg_exp(-cf[{o + 4}] * invT)"
+                fc += f" + cg_exp2(-cf[{o + 4}] * invT)"
             L.append(f"    const double fcent = {fc};")
-            L.append(f"    const double lf = cg_log(fcent) * {dlit(1.0 / LN10)};")
-            L.append(f"    const double lp = cg_log(pr) * {dlit(1.0 / LN10)};")
+            L.append(f"    const double lf = cg_log2(fcent) * {dlit(LOG10_2)};  // log10 Fcent")
+            L.append(f"    const double lp = cg_log2(pr) * {dlit(LOG10_2)};     // log10 Pr")
             L.append("    const double cc = -0.4 - 0.67 * lf;")
             L.append("    const double nn = 0.75 - 1.27 * lf;")
-            L.append("    const double f1 = (lp + cc) / (nn - 0.14 * (lp + cc));")
-            L.append(f"    const double F = cg_exp({dlit(LN10)} * lf / (1.0 + f1 * f1));")
-            L.append("    const double kf = kinf * (pr / (1.0 + pr)) * F;")
+            L.append("    const double f1 = cg_div(lp + cc, nn - 0.14 * (lp + cc));")
+            L.append(f"    const double F = cg_exp2(cg_div({dlit(LOG2_10)} * lf, 1.0 + f1 * f1));")
+            L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr) * F;")
         elif r.rtype == "sri":
-            L.append(f"    const double lp = cg_log(pr) * {dlit(1.0 / LN10)};")
-            L.append("    const double X = 1.0 / (1.0 + lp * lp);")
-            L.append(f"    const double base = cf[{o}] * cg_exp(-cf[{o + 1}] * invT) + cg_exp(-t * cf[{o + 2}]);")
-            L.append(f"    double F = cf[{o + 3}] * cg_exp(X * cg_log(base));")
+            L.append(f"    const double lp = cg_log2(pr) * {dlit(LOG10_2)};")
+            L.append("    const double X = cg_div(1.0, 1.0 + lp * lp);")
+            L.append(f"    const double base = cf[{o}] * cg_exp2(-cf[{o + 1}] * invT) + cg_exp2(-t * cf[{o + 2}]);")
+            L.append(f"    double F = cf[{o + 3}] * cg_exp2(X * cg_log2(base));")
             if len(r.sri) == 5 and r.sri[4] != 0.0:
-                L.append(f"    F *= cg_exp(cf[{o + 4}] * lnT);")
-            L.append("    const double kf = kinf * (pr / (1.0 + pr)) * F;")
+                L.append(f"    F *= cg_exp2(cf[{o + 4}] * lg2T);")
+            L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr) * F;")
         else:
-            L.append("    const double kf = kinf * (pr / (1.0 + pr));")
+            L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr);")
         if r.reversible:
-            L.append(f"    const double kr = kf * cg_exp(-({dg}));")
+            if kr_mode == "eg":
+                L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
+            else:
+                L.append(f"    const double kr = kf * cg_exp2(-({dg}));")
     if r.reversible:
         q = f"kf * {cr} - kr * {cp}"
     else:
@@ -244,15 +294,57 @@ def _eq(r: Reaction) -> str:
     return f"{side(r.reactants)} {'<=>' if r.reversible else '=>'} {side(r.products)} [{r.rtype}]"
 
 
+def reaction_cost(r: Reaction) -> float:
+    """Rough VALU instruction count of one generated reaction (for load balancing)."""
+    n = len(r.reactants) + len(r.products)
+    c = 30.0 + 3 * n + (12.0 if r.reversible else 0.0)
+    if r.rtype != "standard":
+        c += 2 * len(r.third_body) + 2
+    c += {"troe": 120.0, "sri": 110.0, "lindemann": 35.0}.get(r.rtype, 0.0)
+    return c
+
+
+def partition_reactions(mech: Mechanism, nparts: int) -> List[List[int]]:
+    """Split reactions over wavefronts: balance the cost, prefer the part whose reactions
+    already touch the same species (fewer partial-wdot registers per wavefront)."""
+    if nparts <= 1:
+        return [list(range(mech.n_reactions))]
+    cost = [reaction_cost(r) for r in mech.reactions]
+    cap = sum(cost) / nparts * 1.03
+    load = [0.0] * nparts
+    touched = [set() for _ in range(nparts)]
+    parts: List[List[int]] = [[] for _ in range(nparts)]
+    order = sorted(range(mech.n_reactions), key=lambda j: -cost[j])
+    for j in order:
+        r = mech.reactions[j]
+        sp = set(r.reactants) | set(r.products)
+        best = None
+        for v in range(nparts):
+            if load[v] + cost[j] > cap:
+                continue
+            key = (len(sp - touched[v]), load[v])
+            if best is None or key < best[0]:
+                best = (key, v)
+        v = best[1] if best else min(range(nparts), key=lambda u: load[u])
+        parts[v].append(j)
+        load[v] += cost[j]
+        touched[v] |= sp
+    return [sorted(p) for p in parts]
+
+
 def emit_hip(mech: Mechanism, variant: Variant = Variant()):
     """Render the HIP source.  Returns (source, coef, meta dict)."""
+    if variant.split > 1:
+        return _emit_split(mech, variant)
     import jinja2
     coef, off_smh, roffs = build_coef(mech)
     idx = {s.name: i for i, s in enumerate(mech.species)}
     ns = mech.n_species
     lines: List[str] = []
     for j, r in enumerate(mech.reactions):
-        lines.extend(_reaction_code(j, r, roffs[j], idx))
+        if variant.g_lds and j > 0 and j % max(1, variant.group) == 0:
+            lines.append("  CG_FENCE();")
+        lines.extend(_reaction_code(j, r, roffs[j], idx, variant.kr_mode))
     tmids = []
     group_of = []
     for s in mech.species:
@@ -269,12 +361,58 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
         off_invmw=0, off_smh=off_smh,
         csum_expr=tree_sum([f"c{k}" for k in range(ns)]),
         ctot_expr=tree_sum([f"c{k}" for k in range(ns)]),
-        need_ctot=need_ctot, R0=dlit(R0), ln_patm_r0=dlit(math.log(PATM / R0)),
+        need_ctot=need_ctot, R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)),
+        patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
+        exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
+        g_lds=variant.g_lds and variant.kr_mode == "eg",
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of,
-        reactions_code="\n".join(lines),
+        reactions_code="\n".join(lines), threads=variant.threads, ppb=variant.ppb,
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h)
+    return src, coef, meta
+
+
+def _emit_split(mech: Mechanism, variant: Variant):
+    import jinja2
+    coef, off_smh, roffs = build_coef(mech)
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    ns = mech.n_species
+    W = variant.split
+    parts = partition_reactions(mech, W)
+    codes, touched = [], []
+    for v in range(W):
+        lines: List[str] = []
+        tv = set()
+        for q, j in enumerate(parts[v]):
+            if q > 0 and q % max(1, variant.group) == 0:
+                lines.append("    CG_FENCE();")
+            r = mech.reactions[j]
+            lines.extend("  " + ln for ln in _reaction_code(j, r, roffs[j], idx, variant.kr_mode))
+            tv |= {idx[s] for s in r.reactants} | {idx[s] for s in r.products}
+        codes.append("\n".join(lines))
+        touched.append(sorted(tv))
+    bounds = [round(ns * v / W) for v in range(W + 1)]
+    species_of = [list(range(bounds[v], bounds[v + 1])) for v in range(W)]
+    tmids, group_of = [], []
+    for s in mech.species:
+        if s.thermo.t_mid not in tmids:
+            tmids.append(s.thermo.t_mid)
+        group_of.append(tmids.index(s.thermo.t_mid))
+    env = jinja2.Environment(loader=jinja2.FileSystemLoader(TEMPLATE_DIR), keep_trailing_newline=True)
+    tpl = env.get_template("getrates_split_hip.j2")
+    h = structure_hash(mech)
+    src = tpl.render(
+        name=mech.name, ns=ns, nr=mech.n_reactions, ncoef=len(coef), W=W, threads=variant.threads,
+        ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, hash_hex=f"0x{h:016x}", off_invmw=0,
+        off_smh=off_smh, R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0),
+        r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
+        tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of, species_of=species_of,
+        zero_of=species_of, touched=touched, reactions_code=codes,
+    )
+    meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,
+                waves_per_eu=variant.waves_per_eu, structure_hash=h,
+                touched_per_wave=[len(t) for t in touched])
     return src, coef, meta
 
 

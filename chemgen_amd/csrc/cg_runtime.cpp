@@ -35,7 +35,7 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 struct Meta {
-  int32_t abi, ns, nr, ncoef, block, waves_per_eu;
+  int32_t abi, ns, nr, ncoef, block, ppb;  // block = threads per workgroup, ppb = points per workgroup
   uint64_t structure_hash;
 };
 
@@ -86,7 +86,7 @@ extern "C" int cg_mech_load(const char* path, const double* coef, int64_t n_coef
     delete h;
     return fail(CG_ELOAD, "code object %s: %s", path, hipGetErrorString(e));
   }
-  if (h->meta.abi != 1 || h->meta.ncoef != n_coef ||
+  if (h->meta.abi != 2 || h->meta.ppb <= 0 || h->meta.ncoef != n_coef ||
       (expect_hash != 0 && h->meta.structure_hash != expect_hash)) {
     int code = fail(CG_ELOAD,
                     "code object %s does not match the coefficients (abi %d, ncoef %d vs %lld, "
@@ -125,7 +125,7 @@ extern "C" int cg_mech_info(cg_mech_t h, int32_t* ns, int32_t* nr, int32_t* bloc
   if (!h) return fail(CG_EINVAL, "cg_mech_info: null handle");
   if (ns) *ns = h->meta.ns;
   if (nr) *nr = h->meta.nr;
-  if (block) *block = h->meta.block;
+  if (block) *block = h->meta.ppb;
   if (hash) *hash = h->meta.structure_hash;
   return CG_OK;
 }
@@ -147,7 +147,7 @@ extern "C" int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_
                                      (long long)ng, (long long)ld);
   if (ng == 0) return CG_OK;
   if (!P || !T || !Y || !W) return fail(CG_EINVAL, "cg_getrates: null array");
-  const int64_t nb = (ng + h->meta.block - 1) / h->meta.block;
+  const int64_t nb = (ng + h->meta.ppb - 1) / h->meta.ppb;
   if (nb > 0x7fffffffLL) return fail(CG_ERANGE, "cg_getrates: ng too large");
   void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d, (void*)&ng, (void*)&ld};
   CG_HIP(hipModuleLaunchKernel(h->fn, (unsigned)nb, 1, 1, (unsigned)h->meta.block, 1, 1, 0,
@@ -160,7 +160,7 @@ extern "C" int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n
   if (offsets[0] != 0) return fail(CG_EINVAL, "cg_batched_prepare: offsets[0] must be 0");
   // the device tile list may still be read by an earlier batched launch
   if (h->n_tiles > 0) CG_HIP(hipDeviceSynchronize());
-  const int64_t B = h->meta.block;
+  const int64_t B = h->meta.ppb;
   std::vector<int64_t> start;
   std::vector<int32_t> len;
   for (int32_t b = 0; b < n_patches; ++b) {

@@ -17,6 +17,14 @@ def load_bundled(name: str) -> Mechanism:
     return parse_chemkin(os.path.join(DATA_DIR, name + ".inp"), name=name)
 
 
+TUNED = ("synth_gri53", "synth_nhept88", "gri30")
+
+
 def default_variants(mech: Mechanism) -> List[Variant]:
-    """Variants compiled by the build: the default shape first."""
-    return [Variant(64, 1)]
+    """Variants compiled by the build: the tuned (or default) shape first, then the
+    autotuner's candidates for the mechanisms it sweeps."""
+    from .autotune import best_variant, candidate_variants
+    out = [best_variant(mech)]
+    if mech.name in TUNED:
+        out += [v for v in candidate_variants(mech) if v not in out]
+    return out

@@ -95,10 +95,10 @@ class DeviceMechanism:
     """One mechanism loaded on the current GPU (code object + gpu_coef blob)."""
 
     def __init__(self, mech: Mechanism, variant: Variant = Variant(), build: bool = True,
-                 coef: Optional[np.ndarray] = None):
+                 coef: Optional[np.ndarray] = None, code_object: Optional[str] = None):
         self.mech = mech
         self.variant = variant
-        self.path = ensure_code_object(mech, variant, build=build)
+        self.path = code_object or ensure_code_object(mech, variant, build=build)
         c, _, _ = build_coef(mech)
         if coef is not None:
             c = np.asarray(coef, dtype=np.float64)
@@ -124,8 +124,8 @@ class DeviceMechanism:
         """In-place, on device tensors: P[ng], T[ng], Y[Ns, ld], wdot[Ns, ld] (float64)."""
         ng = int(P.shape[0])
         _check_dev(P, T, Y, wdot, self.n_species, ng)
-        ld = int(Y.stride(0))
-        if int(wdot.stride(0)) != ld:
+        ld = _ld(Y)
+        if _ld(wdot) != ld:
             raise ChemgenError("Y and wdot must share the leading dimension")
         check(lib().cg_getrates(self._h, P.data_ptr(), T.data_ptr(), ng, ld, Y.data_ptr(),
                                 wdot.data_ptr(), _stream_ptr(stream)), "cg_getrates")
@@ -142,7 +142,9 @@ class DeviceMechanism:
         off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
         ng = int(off[-1])
         _check_dev(P, T, Y, wdot, self.n_species, ng)
-        ld = int(Y.stride(0))
+        ld = _ld(Y)
+        if _ld(wdot) != ld:
+            raise ChemgenError("Y and wdot must share the leading dimension")
         check(lib().cg_getrates_batched(self._h, P.data_ptr(), T.data_ptr(), Y.data_ptr(),
                                         wdot.data_ptr(), ld, off.ctypes.data, len(off) - 1, grid,
                                         _stream_ptr(stream)), "cg_getrates_batched")
@@ -152,7 +154,7 @@ class DeviceMechanism:
         check(lib().cg_batched_prepare(self._h, off.ctypes.data, len(off) - 1), "cg_batched_prepare")
 
     def getrates_batched_prepared_(self, P, T, Y, wdot, grid: int = 0, stream=None) -> None:
-        ld = int(Y.stride(0))
+        ld = _ld(Y)
         check(lib().cg_getrates_batched_prepared(self._h, P.data_ptr(), T.data_ptr(), Y.data_ptr(),
                                                  wdot.data_ptr(), ld, grid, _stream_ptr(stream)),
               "cg_getrates_batched_prepared")
@@ -179,5 +181,12 @@ def _check_dev(P, T, Y, wdot, ns, ng):
     if not (P.is_contiguous() and T.is_contiguous()):
         raise ChemgenError("P and T must be contiguous")
     for name, t in (("Y", Y), ("wdot", wdot)):
-        if t.dim() != 2 or t.shape[0] != ns or t.stride(1) != 1 or t.shape[1] < ng:
+        if t.dim() != 2 or t.shape[0] != ns or t.shape[1] < ng or (t.shape[1] > 1 and t.stride(1) != 1):
             raise ChemgenError(f"{name} must be species-major [{ns}, >=ng] with unit point stride")
+        if ns > 1 and t.stride(0) < t.shape[1]:
+            raise ChemgenError(f"{name}: species rows overlap (stride {t.stride(0)} < {t.shape[1]})")
+
+
+def _ld(t) -> int:
+    """Leading dimension (distance between species rows) of a [Ns, n] tensor."""
+    return int(t.stride(0)) if t.shape[0] > 1 else int(t.shape[1])
