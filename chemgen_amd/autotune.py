@@ -24,11 +24,11 @@ TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "
 
 def candidate_variants(mech: Mechanism) -> List[Variant]:
     """Candidates swept by default (all compiled by the build for the bundled mechanisms)."""
-    out = [Variant(64, 1, "lean", "eg", "horner", False),   # lane per point, state in registers
-           Variant(64, 2, "lean", "eg", "horner", True, 4)]  # lane per point, EG in LDS
+    out = [Variant(64, 1, "lean", "gh", "horner", False),   # lane per point, state in registers
+           Variant(64, 2, "lean", "gh", "horner", True, 4)]  # lane per point, EG in LDS
     for split, waves in ((4, 2), (4, 3), (6, 3), (8, 3), (8, 4)):   # reaction-split tiles
         for group in (4, 8):
-            out.append(Variant(64, waves, "lean", "eg", "horner", False, group, split))
+            out.append(Variant(64, waves, "lean", "gh", "horner", False, group, split))
     return out
 
 
@@ -81,9 +81,12 @@ def tune(mech: Mechanism, ng: int = 1 << 20, variants: Optional[List[Variant]] =
             ref = w
             agree = 0.0
         else:
-            scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1e-300)
+            # difference relative to the largest |wdot| of the same grid point: species near
+            # steady state have |wdot| far below the terms summed into it, so a per-species
+            # scale would flag harmless reordering; a wrong kernel is off by O(1) here
+            scale = np.maximum(np.abs(ref).max(axis=0, keepdims=True), 1e-300)
             agree = float(np.max(np.abs(w - ref) / scale))
-        ok = bool(np.isfinite(w).all()) and agree <= 1e-10
+        ok = bool(np.isfinite(w).all()) and agree <= 1e-9
         results.append({"variant": dataclasses.asdict(v), "tag": v.tag, "ms": ms,
                         "points_per_s": ng / (ms / 1e3), "agree": agree, "ok": ok})
         print(json.dumps(results[-1]), flush=True)

@@ -66,7 +66,9 @@ class Variant:
     block: int = 64
     waves_per_eu: int = 1
     exp_impl: str = "lean"   # "lean": own exp/log (range reduction + polynomials); "ocml": libm
-    kr_mode: str = "eg"      # "eg": kb from per-species exp(smh) products; "exp": one exp per kb
+    kr_mode: str = "gh"      # "gh": q = kf prod(g_r) (prod(h_r) - pfac^-dnu prod(h_p)) with
+                             #   g = exp(smh), h = C/g per species (no per-reaction division);
+                             # "eg": kb from g products and one division; "exp": one exp per kb
     poly: str = "horner"     # exp polynomial evaluation: "horner" or "estrin" (shorter chains)
     g_lds: bool = False      # keep the per-species EG_k in LDS instead of registers
     group: int = 4           # with LDS state: reactions between re-reads from LDS
@@ -202,10 +204,24 @@ def _kr_eg(r: Reaction, idx: Dict[str, int], kf: str) -> str:
     return f"cg_div_fast({kf} * {num}, {den}){_pfac_pow(r.coeff_sum)}"
 
 
+def _pf_factor(dnu: float) -> str:
+    """pfac^(-dnu) as one expression ("" for dnu = 0)."""
+    if dnu == 0.0:
+        return ""
+    if float(dnu).is_integer() and abs(dnu) <= 3:
+        base = "pinv" if dnu > 0 else "pfac"
+        return "(" + " * ".join([base] * int(abs(dnu))) + ")"
+    return f"cg_exp2({dlit(-dnu)} * lg2pfac)"
+
+
 def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp") -> List[str]:
     L = [f"  {{  // R{j}: {_eq(r)}"]
-    cr = stoich_product(power_expr(f"c{idx[s]}", n) for s, n in r.reactants.items())
-    cp = stoich_product(power_expr(f"c{idx[s]}", n) for s, n in r.products.items())
+    if kr_mode == "gh":
+        cvar = lambda k: f"(g{k} * h{k})"
+    else:
+        cvar = lambda k: f"c{k}"
+    cr = stoich_product(power_expr(cvar(idx[s]), n) for s, n in r.reactants.items())
+    cp = stoich_product(power_expr(cvar(idx[s]), n) for s, n in r.products.items())
     # Delta(smh) + dnu*ln(pfac): kb = kf * exp(-(sum_p nu smh - sum_r nu smh) - dnu ln pfac)
     dg_terms = []
     for s, n in r.products.items():
@@ -222,7 +238,7 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
     if r.rtype in ("standard", "third_body"):
         L.append(f"    const double lk = {_lk(off, r.arrh[1], r.arrh[2])};")
         L.append("    const double kf = cg_exp2_fin(lk);")
-        if r.reversible:
+        if r.reversible and kr_mode != "gh":
             if kr_mode == "eg":
                 L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
             else:
@@ -238,7 +254,7 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
         # M = ctot + sum (eff-1) C, in the order the efficiencies are listed
         M = "ctot"
         for q, s in enumerate(r.third_body):
-            M = f"__builtin_fma(cf[{tb_off + q}], c{idx[s]}, {M})"
+            M = f"__builtin_fma(cf[{tb_off + q}], {cvar(idx[s])}, {M})"
         L.append(f"    const double M = {M};")
     if r.rtype in ("troe", "lindemann", "sri"):
         L.append(f"    const double lki = {_lk(off, r.arrh[1], r.arrh[2])};")
@@ -268,12 +284,24 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
             L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr) * F;")
         else:
             L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr);")
-        if r.reversible:
+        if r.reversible and kr_mode != "gh":
             if kr_mode == "eg":
                 L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
             else:
                 L.append(f"    const double kr = kf * cg_exp2(-({dg}));")
-    if r.reversible:
+    if kr_mode == "gh":
+        # q = kf prod(C_r) - kb prod(C_p) with C = g h and kb = kf prod(g_r)/prod(g_p) pfac^-dnu:
+        # q = kf prod(g_r) (prod(h_r) - pfac^-dnu prod(h_p))
+        gr = stoich_product(power_expr(f"g{idx[s]}", n) for s, n in r.reactants.items())
+        hr = stoich_product(power_expr(f"h{idx[s]}", n) for s, n in r.reactants.items())
+        hp = stoich_product(power_expr(f"h{idx[s]}", n) for s, n in r.products.items())
+        if r.reversible:
+            pf = _pf_factor(r.coeff_sum)
+            diff = f"__builtin_fma(-{pf}, {hp}, {hr})" if pf else f"({hr} - {hp})"
+            q = f"(kf * {gr}) * {diff}"
+        else:
+            q = f"(kf * {gr}) * ({hr})"
+    elif r.reversible:
         q = f"kf * {cr} - kr * {cp}"
     else:
         q = f"kf * {cr}"
@@ -364,7 +392,7 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
         need_ctot=need_ctot, R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)),
         patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
         exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
-        g_lds=variant.g_lds and variant.kr_mode == "eg",
+        g_lds=variant.g_lds and variant.kr_mode in ("eg", "gh"),
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of,
         reactions_code="\n".join(lines), threads=variant.threads, ppb=variant.ppb,
     )
