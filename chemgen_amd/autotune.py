@@ -97,6 +97,9 @@ def tune(mech: Mechanism, ng: int = 1 << 20, variants: Optional[List[Variant]] =
              "best": best["variant"] if best else dataclasses.asdict(Variant()), "results": results}
     if record and best:
         db = _load_db()
+        if out_path and os.path.exists(out_path):
+            with open(out_path) as fh:
+                db = json.load(fh)
         db[mech.name] = entry
         with open(out_path or TUNING_PATH, "w") as fh:
             json.dump(db, fh, indent=1, sort_keys=True)
