@@ -1,0 +1,138 @@
+"""Command line of the generator, with the reference's flags (generate_chemistry.py:7-54).
+
+    python -m chemgen_amd.generate_chemistry --mode getrates --language hip \
+        --mech chem.inp --therm therm.dat --output getrates_hip.py [--parallel-level N]
+    python -m chemgen_amd.generate_chemistry --mode gpu_coef --mech chem.inp --output DIR
+
+``--mode getrates --language hip`` writes, next to ``--output``:
+  <stem>.py          Python module exposing ``wdot = getrates(P, T, Y)``
+  <stem>.hip         the generated gfx950 kernel source
+  <stem>.hsaco       its code object (hipcc --genco)
+  <stem>.coef.npy    the gpu_coef blob
+  <stem>.json, .inp  mechanism metadata and the mechanism as parsed
+``--mode gpu_coef`` writes DIR/gpu_coef_cdna4.npy (+ .json): the coefficient blob the
+kernels read (the layout is documented in chemgen_amd/codegen.py).
+
+Launch shape, the analogue of the reference's parallel levels (--parallel-level,
+--veclen, --nreact-per-block):
+  1  one lane per grid point, VECLEN points per workgroup (rounded up to 64)
+  2  reaction-split tiles: NREACT_PER_BLOCK wavefronts share a 64-point tile in LDS
+  3  as 2 with 3 waves per SIMD requested from the register allocator
+  4  the autotuned shape for this mechanism (data/tuning.json), else the default
+
+The reference's python/fortran/OpenMP targets, its Fortran coefficient module and the
+mini-app are not produced here (those targets exist in the reference).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+
+def parse_arguments(argv=None):
+    p = argparse.ArgumentParser(description="Generate GPU coefficients and getrates functions")
+    p.add_argument("--mode", choices=["gpu_coef", "getrates", "mini_app"], required=True)
+    p.add_argument("--mech")
+    p.add_argument("--therm")
+    p.add_argument("--yaml_file")
+    p.add_argument("--output", required=True)
+    p.add_argument("--parallel-level", type=int, choices=[1, 2, 3, 4], default=4)
+    p.add_argument("--veclen", type=int, default=64)
+    p.add_argument("--nreact-per-block", type=int, default=4)
+    p.add_argument("--language", choices=["python", "fortran", "hip"], default="hip")
+    p.add_argument("--omp", action="store_true")
+    p.add_argument("--vector", action="store_true")
+    p.add_argument("--module", action="store_true")
+    p.add_argument("--input-mw", action="store_true")
+    p.add_argument("--rtypes-together", action="store_true")
+    p.add_argument("--ng", type=int, default=1)
+    p.add_argument("--ncpu", type=int, default=64)
+    p.add_argument("--ngpu", type=int, default=8)
+    p.add_argument("--nt", type=int, default=1)
+    p.add_argument("--time-cpu", action="store_true")
+    p.add_argument("--compat", choices=["full", "reference"], default="full",
+                   help="reference: keep only the reaction types the reference handles")
+    return p.parse_args(argv)
+
+
+def variant_for(args, mech):
+    from .autotune import best_variant
+    from .codegen import Variant
+    lvl = args.parallel_level
+    if lvl == 1:
+        return Variant(block=max(64, (args.veclen + 63) // 64 * 64), waves_per_eu=1)
+    if lvl in (2, 3):
+        return Variant(64, 2 if lvl == 2 else 3, "lean", "gh", "horner", False, 8,
+                       max(2, args.nreact_per_block))
+    return best_variant(mech)
+
+
+def _load(args):
+    from .mechanism import load_mechanism
+    return load_mechanism(args.mech, args.therm, args.yaml_file, compat=args.compat)
+
+
+def generate_gpu_coefficients(args) -> None:
+    from .codegen import build_coef, structure_hash
+    mech = _load(args)
+    os.makedirs(args.output, exist_ok=True)
+    coef, off_smh, roffs = build_coef(mech)
+    np.save(os.path.join(args.output, "gpu_coef_cdna4.npy"), coef)
+    with open(os.path.join(args.output, "gpu_coef_cdna4.json"), "w") as fh:
+        json.dump({"mechanism": mech.name, "species": mech.species_names, "n_coef": len(coef),
+                   "smh_offset": off_smh, "reaction_offsets": roffs,
+                   "structure_hash": f"{structure_hash(mech):016x}"}, fh, indent=1)
+    print(f"gpu_coef written to {args.output}")
+
+
+def generate_getrates(args) -> None:
+    if args.language != "hip":
+        raise SystemExit("this generator emits --language hip; the python/fortran targets are the "
+                         "reference's own")
+    from .api import save_generated
+    from .codegen import compile_code_object, emit_hip
+    mech = _load(args)
+    v = variant_for(args, mech)
+    stem = os.path.splitext(args.output)[0]
+    os.makedirs(os.path.dirname(os.path.abspath(stem)), exist_ok=True)
+    src, _, _ = emit_hip(mech, v)
+    with open(stem + ".hip", "w") as fh:
+        fh.write(src)
+    compile_code_object(stem + ".hip", stem + ".hsaco")
+    save_generated(stem, mech, v, stem + ".hsaco")
+    base = os.path.basename(stem)
+    with open(stem + ".py", "w") as fh:
+        fh.write(f'''"""Generated by chemgen_amd: getrates for mechanism '{mech.name}'
+({mech.n_species} species, {mech.n_reactions} reactions), FP64 HIP kernel for gfx950."""
+import os
+
+from chemgen_amd.api import load_generated
+
+_impl = load_generated(os.path.join(os.path.dirname(os.path.abspath(__file__)), "{base}"))
+species = _impl.species
+
+
+def getrates(P, T, Y):
+    """wdot = getrates(P, T, Y): P [dyn/cm^2], T [K], Y mass fractions -> wdot [mol/cm^3/s]."""
+    return _impl(P, T, Y)
+''')
+    print(f"Expressions and getrates function written to {args.output}")
+
+
+def main(argv=None) -> None:
+    args = parse_arguments(argv)
+    if args.mode == "gpu_coef":
+        generate_gpu_coefficients(args)
+    elif args.mode == "getrates":
+        generate_getrates(args)
+    else:
+        raise SystemExit("--mode mini_app: the Fortran mini-app is the reference's own harness; "
+                         "use bench.py for timing")
+
+
+if __name__ == "__main__":
+    main()

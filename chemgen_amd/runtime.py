@@ -34,6 +34,10 @@ def lib():
     global _lib
     with _lib_lock:
         if _lib is None:
+            # PyTorch-ROCm ships its own libamdhip64.so.7; load it first so that this library
+            # binds to the same HIP runtime (one runtime per process: torch's streams and
+            # allocations are then valid handles here)
+            import torch  # noqa: F401
             if not os.path.exists(LIB_PATH):
                 raise ChemgenError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
             L = ctypes.CDLL(LIB_PATH)
