@@ -58,13 +58,19 @@ struct cg_mech {
 
 extern "C" const char* cg_last_error(void) { return g_err.c_str(); }
 
+// error reporting for the other translation units of the library
+extern "C" int cg_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
 extern "C" int cg_mech_load(const char* path, const double* coef, int64_t n_coef,
                             uint64_t expect_hash, cg_mech_t* out) {
   if (!path || !coef || !out || n_coef <= 0) return fail(CG_EINVAL, "cg_mech_load: bad argument");
   *out = nullptr;
+  int dev = 0, n_cu = 0;
+  CG_HIP(hipGetDevice(&dev));
+  CG_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
   cg_mech* h = new cg_mech();
-  CG_HIP(hipGetDevice(&h->device));
-  CG_HIP(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
+  h->device = dev;
+  h->n_cu = n_cu;
   hipError_t e = hipModuleLoad(&h->mod, path);
   if (e != hipSuccess) {
     delete h;

@@ -86,6 +86,48 @@ int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const double* T,
                                  const double* Y, double* wdot, int64_t ld, int32_t grid,
                                  void* stream);
 
+/* ---- learned surrogate (no counterpart in the reference: chemgen/chemistry_nn.py:15-85 is a
+ * non-functional prototype).  BF16 values are passed as void* (16-bit, device memory). ---- */
+
+/* C[M,N] = A B (+ bias) on the matrix cores, FP32 accumulation.  A(i,k) = A[i*sAi + k*sAk],
+ * B(k,j) = B[k*sBk + j*sBj] (BF16).  Outputs: C32 (FP32, optional) gets acc + bias; Cb (BF16,
+ * optional) gets act ? SiLU(acc + bias) : acc + bias.  ksplit > 1 splits K over workgroups
+ * and needs accumulate = 1 (C32 += partial, atomically). */
+int cg_mlp_gemm(const void* A, int64_t sAi, int64_t sAk, const void* B, int64_t sBk, int64_t sBj,
+                int32_t M, int32_t N, int32_t K, int32_t ksplit, const float* bias, int32_t act,
+                float* C32, int64_t ldc, void* Cb, int64_t ldcb, int32_t accumulate, void* stream);
+/* X[m, :] = ((T-f0)*f1, (ln P - f2)*f3, (ln(Y_k + 1e-12) - f4)*f5 ..., 0...) in BF16 rows of
+ * width kpad (fscale holds f0..f5) */
+int cg_mlp_features(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng,
+                    int32_t ns, int32_t kpad, const float* fscale, void* X, void* stream);
+/* Z[m, k] = asinh(wdot[k, m] / s[k]) */
+int cg_mlp_targets(const double* W, int64_t ld, int64_t ng, int32_t ns, const double* s, float* Z,
+                   void* stream);
+/* G = 2 scale (Zh - Z) (BF16), *loss += scale sum (Zh - Z)^2 */
+int cg_mlp_mse_grad(const float* Zh, const float* Z, int64_t n, float scale, void* G, float* loss,
+                    void* stream);
+/* dZ = dH * SiLU'(Zp) (BF16) */
+int cg_mlp_silu_back(const float* dH, const float* Zp, int64_t n, void* dZ, void* stream);
+/* db[n] += sum_m G[m, n] (G BF16 [M, N]) */
+int cg_mlp_colsum(const void* G, int64_t M, int32_t N, float* db, void* stream);
+/* Adam step on FP32 master weights w (bias-corrected, step >= 1); w16 (BF16) refreshed */
+int cg_mlp_adam(float* w, const float* g, float* m1, float* m2, int64_t n, float lr, float b1,
+                float b2, float eps, int32_t step, void* w16, void* stream);
+int cg_mlp_to_bf16(const float* x, int64_t n, void* y, void* stream);
+/* wdot[k, m] = s[k] sinh(Zh[m, k]) (FP64) and the element-conservation residual
+ * resid[m] = max_e |sum_k a[e,k] wdot_k| / sum_k |a[e,k] wdot_k|; flag[m] = resid > tol */
+int cg_mlp_decode(const float* Zh, int64_t ng, int32_t ns, const double* s, const float* aek,
+                  int32_t ne, double tol, double* W, int64_t ld, int32_t* flag, double* resid,
+                  void* stream);
+/* idx[(*count)++] = m for every flagged m (order unspecified) */
+int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t* count, int64_t* idx, void* stream);
+/* pack / unpack the listed points (first *count of idx; cap = launch bound) */
+int cg_mlp_gather(const double* P, const double* T, const double* Y, int64_t ld, const int64_t* idx,
+                  const int32_t* count, int64_t cap, int32_t ns, double* Po, double* To, double* Yo,
+                  int64_t ldo, void* stream);
+int cg_mlp_scatter(const double* Wi, int64_t ldi, const int64_t* idx, const int32_t* count, int64_t cap,
+                   int32_t ns, double* W, int64_t ld, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
