@@ -1,0 +1,118 @@
+"""Train and evaluate the getrates surrogate on the GPU.
+
+    python -m chemgen_amd.surrogate_train --mech synth_gri53 --n-train 8388608 --epochs 12 \
+        --out chemgen_amd/data/surrogate_synth_gri53.npz --report report.json
+
+Training states: T U(800, 2800) K, P 1-40 atm log-uniform, Y Dirichlet(0.5) (seeded, drawn on
+the GPU), labelled by the FP64 getrates kernel.  Evaluation: 2^20 held-out states (another
+seed): per-point error max_k |w_sur - w| / (|w| + s_k) with s_k = 1e-3 x p99 |w_k|, before
+and after the fallback, the fraction of points that fell back, and the throughput of
+surrogate + fallback against the FP64 kernel on the same points.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+
+def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: int = 64, reps: int = 5):
+    import torch
+    from .surrogate import random_states
+    P, T, Y = random_states(sur.ns, n_eval, seed=seed)
+    ref = torch.empty_like(Y)
+    dm.getrates_(P, T, Y, ref)
+    W = torch.empty_like(Y)
+    flag, resid = sur.predict_(P, T, Y, W, tol)
+    err_raw = ((W - ref).abs() / (ref.abs() + sur.s[:, None])).max(dim=0).values
+    info = sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every)
+    err = ((W - ref).abs() / (ref.abs() + sur.s[:, None])).max(dim=0).values
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3, (time.perf_counter() - t0) / reps
+
+    t_fp64, _ = timed(lambda: dm.getrates_(P, T, Y, ref))
+    t_sur, wall_sur = timed(lambda: sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every))
+    X = sur.features(P, T, Y)
+    t_mlp, _ = timed(lambda: sur.forward(X))
+    flops = 2.0 * sum(sur.dims[i] * sur.dims[i + 1] for i in range(len(sur.dims) - 1)) * n_eval
+    q = lambda e, p: float(torch.quantile(e[: 1 << 20].float(), p).item())
+    return {
+        "n_eval": n_eval,
+        "surrogate_only": {"frac_le_1pct": float((err_raw <= 1e-2).double().mean()),
+                           "median_err": q(err_raw, 0.5), "p99_err": q(err_raw, 0.99)},
+        "with_fallback": {"frac_le_1pct": float((err <= 1e-2).double().mean()), "median_err": q(err, 0.5),
+                          "p99_err": q(err, 0.99), **info,
+                          "fallback_frac": info["fallback_points"] / n_eval},
+        "residual_flag_frac": float(flag.double().mean()),
+        "fp64_kernel_points_per_s": n_eval / t_fp64,
+        "surrogate_path_points_per_s": n_eval / t_sur,
+        "surrogate_path_wall_points_per_s": n_eval / wall_sur,
+        "speedup_vs_fp64": t_fp64 / t_sur,
+        "mlp_forward_ms": t_mlp * 1e3,
+        "mlp_forward_tflops": flops / t_mlp / 1e12,
+    }
+
+
+def main(argv=None):
+    import numpy as np
+    import torch
+    from .autotune import best_variant
+    from .kernels import load_bundled
+    from .runtime import DeviceMechanism
+    from .surrogate import Surrogate, random_states
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mech", default="synth_gri53")
+    ap.add_argument("--n-train", type=int, default=1 << 23)
+    ap.add_argument("--n-eval", type=int, default=1 << 20)
+    ap.add_argument("--hidden", type=int, default=256)
+    ap.add_argument("--layers", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=None, help="weights (.npz)")
+    ap.add_argument("--report", default=None, help="JSON report")
+    ap.add_argument("--load", default=None, help="skip training, load weights")
+    a = ap.parse_args(argv)
+    mech = load_bundled(a.mech)
+    dm = DeviceMechanism(mech, best_variant(mech), build=False)
+    sur = Surrogate(mech, hidden=a.hidden, n_hidden=a.layers, seed=0)
+    report = {"mechanism": a.mech, "dims": sur.dims, "n_train": a.n_train, "epochs": a.epochs, "batch": a.batch}
+    if a.load:
+        sur.load_state_dict(dict(np.load(a.load)))
+    else:
+        t0 = time.perf_counter()
+        P, T, Y = random_states(mech.n_species, a.n_train, seed=100)
+        W = torch.empty_like(Y)
+        dm.getrates_(P, T, Y, W)
+        torch.cuda.synchronize()
+        t_data = time.perf_counter() - t0
+        sur.fit_scales(W)
+        t0 = time.perf_counter()
+        hist = sur.train(P, T, Y, W, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                         log=lambda s: print("[train]", s, flush=True))
+        torch.cuda.synchronize()
+        report.update({"data_s": t_data, "train_s": time.perf_counter() - t0, "loss_history": hist})
+        del P, T, Y, W
+        torch.cuda.empty_cache()
+        if a.out:
+            np.savez(a.out, **sur.state_dict())
+    report["eval"] = evaluate(sur, dm, a.n_eval, seed=7)
+    print(json.dumps(report["eval"]), flush=True)
+    if a.report:
+        with open(a.report, "w") as fh:
+            json.dump(report, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()
