@@ -36,6 +36,72 @@ def cpu_baseline(mech_name, mech, target_s=10.0, sample_ng=1 << 14):
                       f"the reference's getrates_i, {dt:.1f} s"}
 
 
+def bench_batched(steps, warmup):
+    """Ragged batch on the n-heptane-sized mechanism: 64 patches with log-uniform sizes in
+    2^8..2^17 points, 2^22 points in total, T U(600, 2800) K; the prepared persistent launch
+    is captured in a HIP graph and the graph is replayed and timed."""
+    import numpy as np
+    import torch
+    from chemgen_amd.autotune import best_batched
+    from chemgen_amd.kernels import load_bundled
+    from chemgen_amd.roofline import PEAK_FP64_VECTOR, work_per_point
+    from chemgen_amd.runtime import DeviceMechanism
+    from chemgen_amd.synth import make_states, ragged_patches
+    from oracle.getrates_np import getrates as np_oracle
+    from oracle.metrics import parity_stats
+    name = "synth_nhept88"
+    mech = load_bundled(name)
+    offs = ragged_patches()
+    ng = int(offs[-1])
+    variant, grid = best_batched(mech)
+    P, T, Y = make_states(mech.n_species, ng, seed=0, T_range=(600.0, 2800.0))
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
+    Wd = torch.empty_like(Yd)
+    dm = DeviceMechanism(mech, variant, build=False)
+    dm.batched_prepare(offs)
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            dm.getrates_batched_prepared_(Pd, Td, Yd, Wd, grid=grid, stream=s)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        for _ in range(warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(steps):
+            g.replay()
+        e1.record(s)
+    torch.cuda.synchronize()
+    t_graph = e0.elapsed_time(e1) / steps / 1e3
+    # the same points as one contiguous plain launch, for the cost of raggedness
+    e0.record(s)
+    for _ in range(steps):
+        dm.getrates_(Pd, Td, Yd, Wd, s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    t_plain = e0.elapsed_time(e1) / steps / 1e3
+    g.replay()
+    torch.cuda.synchronize()
+    pick = np.concatenate([np.arange(a, min(a + 4, b)) for a, b in zip(offs[:-1], offs[1:])])
+    Wc = Wd[:, torch.from_numpy(pick).to(dev)].cpu().numpy()
+    ref, qf, qb = np_oracle(mech, P[pick], T[pick], Y[:, pick], return_parts=True)
+    par = parity_stats(mech, Wc, ref, qf, qb)
+    w = work_per_point(mech)
+    sizes = np.diff(offs)
+    return {"workload": f"{name} ({mech.n_species} species, {mech.n_reactions} reactions): ragged batch of "
+                        f"{len(sizes)} patches, log-uniform 2^8-2^17 points, {ng} points in total, "
+                        f"T U(600,2800) K, seed 0; one persistent launch replayed from a HIP graph",
+            "points_per_s": ng / t_graph, "kernel_avg_ms": t_graph * 1e3,
+            "contiguous_points_per_s": ng / t_plain, "kernel_variant": variant.tag,
+            "grid": grid if grid > 0 else ("8 x CUs" if grid == 0 else "one workgroup per tile"), "patch_min": int(sizes.min()), "patch_max": int(sizes.max()),
+            "roofline_frac": w["flops"] * ng / t_graph / PEAK_FP64_VECTOR,
+            "parity": {"sample_points": int(len(pick)), **par}}
+
+
 def _finite(o):
     """JSON has no inf/nan: report them as strings."""
     if isinstance(o, dict):
@@ -56,6 +122,7 @@ def main():
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
     ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
                     help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
@@ -144,6 +211,10 @@ def main():
                      "kernel_avg_ms": avg_launch_s * 1e3},
         "parity": {"sample_points": nchk, **par},
     }
+    if not args.no_batched:
+        del Pd, Td, Yd, Wd
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
     print(json.dumps(_finite(out)))

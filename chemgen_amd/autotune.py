@@ -100,8 +100,84 @@ def tune(mech: Mechanism, ng: int = 1 << 20, variants: Optional[List[Variant]] =
         if out_path and os.path.exists(out_path):
             with open(out_path) as fh:
                 db = json.load(fh)
+        old = db.get(mech.name, {})
+        if "batched" in old and old.get("structure_hash") == entry["structure_hash"]:
+            entry["batched"] = old["batched"]
         db[mech.name] = entry
         with open(out_path or TUNING_PATH, "w") as fh:
+            json.dump(db, fh, indent=1, sort_keys=True)
+    return entry
+
+
+def best_batched(mech: Mechanism):
+    """(variant, grid) tuned for ragged batches of this mechanism; grid 0 = the runtime's
+    default of 8 persistent workgroups per CU."""
+    e = _load_db().get(mech.name, {})
+    b = e.get("batched")
+    if b and e.get("structure_hash") == f"{structure_hash(mech):016x}":
+        return Variant(**b["best"]), int(b["grid"])
+    return best_variant(mech), 0
+
+
+def tune_batched(mech: Mechanism, offsets, P, T, Y, variants: Optional[List[Variant]] = None,
+                 grids=(-1, 0, 1, 2, 4), reps: int = 10, record: bool = True,
+                 out_path: Optional[str] = None) -> Dict:
+    """Time the prepared ragged-batch launch for every (variant, persistent grid) pair on
+    device-resident states P, T, Y (patches laid out back to back, ``offsets`` [n+1]).
+    ``grids`` are workgroups per CU (0 = runtime default of 8 per CU, -1 = one workgroup
+    per tile, the hardware dispatcher then balances the tiles).  All candidates must agree with
+    the first to 1e-9 of each point's largest |wdot|; the fastest is recorded under the
+    mechanism's ``batched`` entry."""
+    import numpy as np
+    import torch
+    from .runtime import DeviceMechanism
+    variants = variants or candidate_variants(mech)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    W = torch.empty_like(Y)
+    ref = None
+    chk = np.unique(np.clip(np.asarray(offsets[:-1]), 0, Y.shape[1] - 1))[:64]
+    results = []
+    for v in variants:
+        dm = DeviceMechanism(mech, v, build=False)
+        dm.batched_prepare(offsets)
+        for gpc in grids:
+            grid = gpc * n_cu if gpc >= 0 else -1
+            print(f"[autotune-batched] {mech.name} {v.tag} grid={grid}", flush=True)
+            dm.getrates_batched_prepared_(P, T, Y, W, grid=grid)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                dm.getrates_batched_prepared_(P, T, Y, W, grid=grid)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            w = W[:, torch.from_numpy(chk).to(W.device)].cpu().numpy()
+            if ref is None:
+                ref, agree = w, 0.0
+            else:
+                scale = np.maximum(np.abs(ref).max(axis=0, keepdims=True), 1e-300)
+                agree = float(np.max(np.abs(w - ref) / scale))
+            ok = bool(np.isfinite(w).all()) and agree <= 1e-9
+            results.append({"variant": dataclasses.asdict(v), "tag": v.tag, "grid_per_cu": gpc, "ms": ms,
+                            "points_per_s": int(offsets[-1]) / (ms / 1e3), "agree": agree, "ok": ok})
+            print(json.dumps(results[-1]), flush=True)
+        dm.close()
+    good = [r for r in results if r["ok"]]
+    best = min(good, key=lambda r: r["ms"]) if good else None
+    entry = {"n_points": int(offsets[-1]), "n_patches": len(offsets) - 1, "results": results,
+             "best": best["variant"] if best else None,
+             "grid": (best["grid_per_cu"] * n_cu if best["grid_per_cu"] >= 0 else -1) if best else 0}
+    if record and best:
+        path = out_path or TUNING_PATH
+        db = _load_db()
+        if out_path and os.path.exists(out_path):
+            with open(out_path) as fh:
+                db = json.load(fh)
+        db.setdefault(mech.name, {"structure_hash": f"{structure_hash(mech):016x}",
+                                  "best": dataclasses.asdict(best_variant(mech))})
+        db[mech.name]["batched"] = entry
+        with open(path, "w") as fh:
             json.dump(db, fh, indent=1, sort_keys=True)
     return entry
 
@@ -113,6 +189,19 @@ if __name__ == "__main__":
     ap.add_argument("mechs", nargs="*", default=["synth_gri53"])
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--out", default=None, help="where to write the tuning database")
+    ap.add_argument("--batched", action="store_true",
+                    help="tune the ragged-batch launch: 64 patches, log-uniform 2^8-2^17 points, 2^22 in total")
     a = ap.parse_args()
     for name in a.mechs:
-        tune(load_bundled(name), ng=a.ng, out_path=a.out)
+        m = load_bundled(name)
+        if a.batched:
+            import torch
+            from .synth import make_states, ragged_patches
+            offs = ragged_patches()
+            P, T, Y = make_states(m.n_species, int(offs[-1]), seed=0, T_range=(600.0, 2800.0))
+            d = torch.device("cuda:0")
+            Pd, Td, Yd = (torch.from_numpy(x).to(d) for x in (P, T, Y))
+            del P, T, Y
+            tune_batched(m, offs, Pd, Td, Yd, out_path=a.out)
+        else:
+            tune(m, ng=a.ng, out_path=a.out)

@@ -24,7 +24,12 @@ constexpr int TN = 64;   // columns of C per workgroup
 constexpr int TK = 32;   // K per step (one 16x16x32 MFMA)
 constexpr int PADK = 40; // LDS row pitch in bf16 (80 B: rows start on different banks)
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware exp2 and reciprocal, z * rcp(1 + 2^(-z log2 e)) (1 ulp-class
+// approximations; the activation is rounded to BF16 right after).  Large |z| stays exact in the
+// limit: 2^(+big) = inf gives z * 0, 2^(-big) = 0 gives z.
+__device__ __forceinline__ float silu(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.44269504088896341f));
+}
 
 // C[M,N] = A[M,K] B[K,N] (+ bias[N]) with A(i,k) = A[i*sAi + k*sAk], B(k,j) = B[k*sBk + j*sBj].
 // 256 threads = 4 wavefronts in a 2x2 arrangement, each owning a 32x32 block of C as 2x2
@@ -114,18 +119,41 @@ gemm_bf16(const bf16* __restrict__ A, int64_t sAi, int64_t sAk, const bf16* __re
 
 // -------- element-wise kernels --------
 
-// x[m, :] = ((T - f0) f1, (ln P - f2) f3, (ln(Y_k + 1e-12) - f4) f5 ..., 0 ...) in BF16, width kpad
+// Feature f of point m: 0 -> (T - f0) f1, 1 -> (ln P - f2) f3, 2..ns+1 -> (ln(Y_k + 1e-12) - f4) f5,
+// ns+2 -> (1000/T - f6) f7, beyond -> 0 (padding).  Split into the load of the state value
+// it depends on and the arithmetic, so that loops can issue many loads before the first use.
+__device__ __forceinline__ double feature_load(int f, const double* __restrict__ P, const double* __restrict__ T,
+                                               const double* __restrict__ Y, int64_t ld, int64_t m, int ns) {
+  if (f == 1) return P[m];
+  if (f == 0 || f == ns + 2) return T[m];
+  if (f < ns + 2) return Y[(int64_t)(f - 2) * ld + m];
+  return 1.0;
+}
+
+// single precision throughout: the features are rounded to BF16 (8 bits) right after
+__device__ __forceinline__ float feature_value(int f, double v, int ns, const float* __restrict__ fs) {
+  const float x = (float)v;
+  if (f == 0) return (x - fs[0]) * fs[1];
+  if (f == 1) return (__logf(x) - fs[2]) * fs[3];
+  if (f < ns + 2) return (__logf(x + 1e-12f) - fs[4]) * fs[5];
+  if (f == ns + 2) return (1000.0f * __builtin_amdgcn_rcpf(x) - fs[6]) * fs[7];
+  return 0.0f;
+}
+
+__device__ __forceinline__ float feature(int f, const double* __restrict__ P, const double* __restrict__ T,
+                                         const double* __restrict__ Y, int64_t ld, int64_t m, int ns,
+                                         const float* __restrict__ fs) {
+  return feature_value(f, feature_load(f, P, T, Y, ld, m, ns), ns, fs);
+}
+
+// x[m, :] = features of point m in BF16, row-major with width kpad
 __global__ void features_k(const double* __restrict__ P, const double* __restrict__ T,
                            const double* __restrict__ Y, int64_t ld, int64_t ng, int ns, int kpad,
                            const float* __restrict__ fscale, bf16* __restrict__ X) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= ng) return;
   bf16* x = X + m * kpad;
-  x[0] = (bf16)(float)((T[m] - fscale[0]) * fscale[1]);
-  x[1] = (bf16)(float)((log(P[m]) - fscale[2]) * fscale[3]);
-  for (int k = 0; k < ns; ++k)
-    x[2 + k] = (bf16)(float)((log(Y[(int64_t)k * ld + m] + 1e-12) - fscale[4]) * fscale[5]);
-  for (int k = ns + 2; k < kpad; ++k) x[k] = (bf16)0.0f;
+  for (int f = 0; f < kpad; ++f) x[f] = (bf16)feature(f, P, T, Y, ld, m, ns, fscale);
 }
 
 // z[m, k] = asinh(wdot[k, m] / s_k)  (training targets, FP32)
@@ -250,6 +278,317 @@ __global__ void scatter_k(const double* __restrict__ Wi, int64_t ldi, const int6
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+
+// ---------------- fused inference ----------------
+//
+// One launch runs the whole network for 16*PT points per wavefront (4 wavefronts per
+// workgroup, one per SIMD) and decodes the result: features -> n_hidden SiLU layers of width
+// H -> output layer -> wdot = s sinh(z) in FP64 -> element-conservation residual -> flag.
+// The activations never leave registers: the network is evaluated transposed,
+// out[n][m] = sum_k W[n][k] h[m][k], with the weights as the MFMA A operand (read from LDS)
+// and the activations as the B operand.  A 16x16 result tile gives lane l the outputs
+// n = 4 (l/16) + v of point l % 16, so the k order inside every 32-wide k-chunk is permuted
+// to match (kperm): chunk c, lane group g, operand element j holds k = kperm(c, g, j), which
+// is exactly what the result tiles 2c and 2c+1 leave in that lane.  The packed weights
+// (cg_mlp_pack) use the same order, so one layer's results feed the next after bias + SiLU +
+// BF16 rounding, with no data movement.
+//
+// The weights stream through LDS in 64 KB chunks (NG output tiles x the layer's k-chunks),
+// double-buffered: while the wavefronts multiply chunk i out of one buffer, chunk i+1 is
+// copied global -> LDS directly (global_load_lds_dwordx4, no registers) into the other.  A
+// 1 KB operand fragment (64 lanes x 16 B) is one such copy instruction, so LDS holds the
+// fragments in exactly the order the MFMA loads read them.
+
+__device__ __forceinline__ int kperm(int c, int g, int j) { return 32 * c + 4 * g + (j < 4 ? j : 12 + j); }
+
+// Wf[((nt * KC + kc) * 64 + lane) * 8 + j] = W[16 nt + lane % 16][kperm(kc, lane / 16, j)], zero
+// outside the N x K matrix W (row-major, BF16).
+__global__ void pack_frag_k(const bf16* __restrict__ W, int N, int K, int NT, int KC, bf16* __restrict__ Wf) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)NT * KC * 512) return;
+  const int j = (int)(e & 7), lane = (int)((e >> 3) & 63);
+  const int64_t f = e >> 9;
+  const int kc = (int)(f % KC), nt = (int)(f / KC);
+  const int n = 16 * nt + (lane & 15), k = kperm(kc, lane >> 4, j);
+  Wf[e] = (n < N && k < K) ? W[(int64_t)n * K + k] : (bf16)0.0f;
+}
+
+// sinh in single precision: the odd series below |z| = 1/8 (relative error < 1e-9 there), the
+// exponential form above; overflow gives +-inf, which the residual test then flags.
+__device__ __forceinline__ float infer_sinh(float z) {
+  const float z2 = z * z;
+  const float e = __builtin_amdgcn_exp2f(z * 1.44269504088896341f);
+  const float big = 0.5f * (e - __builtin_amdgcn_rcpf(e));
+  const float small = z * (1.0f + z2 * (1.0f / 6.0f + z2 * (1.0f / 120.0f)));
+  return fabsf(z) < 0.125f ? small : big;
+}
+
+template <int H>
+struct InferShape {
+  static constexpr int NW = 8;                    // wavefronts per workgroup (two per SIMD)
+  static constexpr int KCH = H / 32;              // k-chunks of a hidden layer
+  static constexpr int NTH = H / 16;              // output tiles of a hidden layer
+  static constexpr int NG = 64 / KCH;             // output tiles per 64 KB weight chunk
+  static constexpr int NCH = NTH / NG;            // chunks of a hidden layer
+  static constexpr int PT = H == 256 ? 2 : 1;     // 16-point tiles per wavefront
+  static constexpr int NP = 16 * PT;              // points per wavefront
+  static constexpr int LPP = 64 / NP;             // lanes per point (features, decode)
+  static constexpr int KP = 64 / (NW * PT);       // feature k-chunks staged per pass (64 KB)
+  static constexpr int TD = 8;                    // output tiles per decode pass
+  static constexpr int ZW = TD * 16 + 1;          // decode buffer pitch (floats)
+  static constexpr int Z_BYTES = NW * NP * ZW * 4;
+  static constexpr int LDS_BYTES = Z_BYTES > 131072 ? Z_BYTES : 131072;
+};
+
+typedef __attribute__((address_space(1))) const void* cg_gptr;
+typedef __attribute__((address_space(3))) void* cg_lptr;
+
+// Issue the direct global -> LDS copy of weight chunk `ci` of the whole network into buffer
+// ci & 1 (fragment f of the chunk -> LDS fragment slot f; wavefront w copies f = w, w+NW, ...).
+// Chunks: layer 0 (NCH of NG x kc0 fragments), hidden layers (NCH of NG x KCH), output
+// (ceil(nt_out / NG) of up to NG x KCH).
+template <int H>
+__device__ __forceinline__ void infer_prefetch(const bf16x8* __restrict__ Wf, int ci, int kc0, int n_hidden,
+                                               int nt_out, bf16x8* wlds) {
+  using S = InferShape<H>;
+  int64_t off;
+  int nfrag;
+  if (ci < S::NCH) {
+    off = (int64_t)ci * S::NG * kc0;
+    nfrag = S::NG * kc0;
+  } else {
+    const int64_t base = (int64_t)S::NTH * kc0;
+    const int c = ci - S::NCH;
+    off = base + (int64_t)c * S::NG * S::KCH;
+    const int co = c - (n_hidden - 1) * S::NCH;   // chunk index within the output layer
+    if (co < 0) {
+      nfrag = S::NG * S::KCH;
+    } else {
+      const int nt = nt_out - co * S::NG;
+      nfrag = (nt < S::NG ? nt : S::NG) * S::KCH;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  bf16x8* dst = wlds + (ci & 1) * 4096;
+  const bf16x8* src = Wf + off * 64;
+  for (int f = wave; f < nfrag; f += S::NW)
+    __builtin_amdgcn_global_load_lds((cg_gptr)(src + f * 64 + lane), (cg_lptr)(dst + f * 64), 16, 0, 0);
+}
+
+// One layer: acc[t][pt] = sum over k of weights x act (no bias), chunk by chunk; chunk `ci`
+// (running index over the network) is in buffer ci & 1 and the next one is prefetched.  MODE
+// fixes which extent is known at compile time, so the MFMA blocks carry no per-fragment
+// branches: 0 = first layer (runtime kc0), 1 = hidden layer, 2 = output layer (runtime nt_out).
+template <int H, int MODE>
+__device__ __forceinline__ void infer_layer(const bf16x8* __restrict__ Wf, int& ci, int n_chunks, int kc0,
+                                            int n_hidden, int nt_out, bf16x8* __restrict__ wlds,
+                                            const float* __restrict__ bl,
+                                            const bf16x8 (&act)[H / 32][InferShape<H>::PT],
+                                            f32x4 (&acc)[H / 16][InferShape<H>::PT]) {
+  using S = InferShape<H>;
+  const int lane = threadIdx.x & 63;
+  const int kc_n = MODE == 0 ? kc0 : S::KCH;
+  const int nt_n = MODE == 2 ? nt_out : S::NTH;
+  // hidden layers start from their bias (rows 16 t + 4 g + v of this lane); the output layer
+  // from zero, its bias is added in the decode
+#pragma unroll
+  for (int t = 0; t < S::NTH; ++t) {
+    const f32x4 b = MODE == 2 ? f32x4{0.f, 0.f, 0.f, 0.f} : *(const f32x4*)(bl + 16 * t + 4 * (lane >> 4));
+#pragma unroll
+    for (int pt = 0; pt < S::PT; ++pt) acc[t][pt] = b;
+  }
+#pragma unroll
+  for (int c0 = 0; c0 < S::NTH; c0 += S::NG) {
+    if (c0 < nt_n) {
+      __builtin_amdgcn_s_waitcnt(0);   // this wavefront's copies of chunk ci have landed
+      __syncthreads();                 // ... everyone's, and chunk ci-1's buffer is free
+      if (ci + 1 < n_chunks) infer_prefetch<H>(Wf, ci + 1, kc0, n_hidden, nt_out, wlds);
+      const bf16x8* __restrict__ buf = wlds + (ci & 1) * 4096;
+      // Steps of Q tiles x one k-chunk; the fragments of step st+1 are loaded (into the other
+      // half of F) before the MFMAs of step st are issued -- sched_barrier keeps that order --
+      // so LDS latency hides behind Q x PT matrix instructions.  The output layer multiplies
+      // all NG tiles of a chunk; tiles past nt_out hold stale LDS data and their results are
+      // never read.
+      constexpr int Q = 4, NQ = S::NG / Q;
+      const int n_steps = kc_n * NQ;
+      bf16x8 F[2][Q];
+#pragma unroll
+      for (int j = 0; j < Q; ++j) F[0][j] = buf[j * kc_n * 64 + lane];
+#pragma unroll
+      for (int st = 0; st < S::KCH * NQ; ++st) {
+        const int kc = st / NQ, t0 = (st % NQ) * Q;
+        if (MODE != 0 || st < n_steps) {
+          if (st + 1 < S::KCH * NQ && (MODE != 0 || st + 1 < n_steps)) {
+            const int kn = (st + 1) / NQ, tn = ((st + 1) % NQ) * Q;
+#pragma unroll
+            for (int j = 0; j < Q; ++j) F[(st + 1) & 1][j] = buf[((tn + j) * kc_n + kn) * 64 + lane];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int j = 0; j < Q; ++j)
+#pragma unroll
+            for (int pt = 0; pt < S::PT; ++pt)
+              acc[c0 + t0 + j][pt] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[st & 1][j], act[kc][pt], acc[c0 + t0 + j][pt], 0, 0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      ++ci;
+    }
+  }
+}
+
+// SiLU + BF16 rounding (the bias is already in acc): a hidden layer's results become the next
+// layer's operand
+template <int H>
+__device__ __forceinline__ void infer_activate(const f32x4 (&acc)[H / 16][InferShape<H>::PT],
+                                               bf16x8 (&act)[H / 32][InferShape<H>::PT]) {
+  using S = InferShape<H>;
+#pragma unroll
+  for (int kc = 0; kc < S::KCH; ++kc) {
+#pragma unroll
+    for (int pt = 0; pt < S::PT; ++pt) {
+      bf16x8 x;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        x[v] = (bf16)silu(acc[2 * kc][pt][v]);
+        x[4 + v] = (bf16)silu(acc[2 * kc + 1][pt][v]);
+      }
+      act[kc][pt] = x;
+    }
+  }
+}
+
+// Wf: packed layers back to back (layer 0: NTH x kc0 fragments, hidden layers: NTH x KCH,
+// output: nt_out x KCH); bias: hidden layers H each, then the output layer's ns values.
+template <int H>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+mlp_infer_k(const double* __restrict__ P, const double* __restrict__ T, const double* __restrict__ Y,
+            int64_t ld, int64_t ng, int ns, const float* __restrict__ fs, const bf16x8* __restrict__ Wf,
+            const float* __restrict__ bias, int kc0, int n_hidden, int nt_out, const double* __restrict__ s,
+            const float* __restrict__ aek, int ne, double tol, double* __restrict__ W, int64_t ldw,
+            int* __restrict__ flag, double* __restrict__ resid) {
+  using S = InferShape<H>;
+  constexpr int PT = S::PT, NP = S::NP, LPP = S::LPP;
+  extern __shared__ __attribute__((aligned(16))) bf16x8 wlds[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t base = ((int64_t)blockIdx.x * S::NW + wave) * NP;
+  const int p = lane % NP, part = lane / NP;  // decode / feature roles: point p, share `part`
+  const int64_t m = base + p;
+  const int64_t mc = m < ng ? m : ng - 1;
+  const int n_chunks = S::NCH * n_hidden + (nt_out + S::NG - 1) / S::NG;
+  int ci = 0;
+  infer_prefetch<H>(Wf, 0, kc0, n_hidden, nt_out, wlds);   // lands in buffer 0 meanwhile
+
+  // 1. features, staged in operand order through this wavefront's part of buffer 1, KP
+  //    k-chunks per pass
+  bf16x8 act[S::KCH][PT];
+  const bf16x8 zero8 = {};
+  bf16* xl = (bf16*)(wlds + 4096 + wave * (S::KP * PT * 64));
+  for (int k_lo = 0; k_lo < kc0; k_lo += S::KP) {
+    const int k_hi = (k_lo + S::KP) < kc0 ? (k_lo + S::KP) : kc0;
+    __syncthreads();
+    // eight loads in flight per lane before the first logarithm (one wavefront per SIMD:
+    // nothing else would hide their latency)
+    for (int f0 = 32 * k_lo + part; f0 < 32 * k_hi; f0 += 8 * LPP) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int f = f0 + u * LPP;
+        v[u] = f < 32 * k_hi ? feature_load(f, P, T, Y, ld, mc, ns) : 1.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int f = f0 + u * LPP;
+        if (f < 32 * k_hi) {
+          const int kc = (f >> 5) - k_lo, q = f & 31;
+          const int fg = (q >> 2) & 3, j = (q & 3) | ((q >> 4) << 2);
+          xl[(((kc * PT + (p >> 4)) * 64 + 16 * fg + (p & 15)) << 3) + j] = (bf16)feature_value(f, v[u], ns, fs);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kc = 0; kc < S::KCH; ++kc)
+      if (kc >= k_lo && kc < k_hi)
+#pragma unroll
+        for (int pt = 0; pt < PT; ++pt) act[kc][pt] = ((const bf16x8*)xl)[((kc - k_lo) * PT + pt) * 64 + lane];
+  }
+#pragma unroll
+  for (int kc = 0; kc < S::KCH; ++kc)
+    if (kc >= kc0)
+#pragma unroll
+      for (int pt = 0; pt < PT; ++pt) act[kc][pt] = zero8;
+
+  // 2. layers
+  f32x4 acc[S::NTH][PT];
+  infer_layer<H, 0>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, bias, act, acc);
+  infer_activate<H>(acc, act);
+  for (int l = 1; l < n_hidden; ++l) {
+    infer_layer<H, 1>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, bias + l * H, act, acc);
+    infer_activate<H>(acc, act);
+  }
+  infer_layer<H, 2>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, nullptr, act, acc);
+  const float* bl = bias + n_hidden * H;
+
+  // 3. decode through LDS, TD output tiles at a time: point p's z values are read by LPP lanes
+  float* zl = (float*)wlds + wave * (NP * S::ZW);
+  float num[8], den[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) { num[e] = 0.0f; den[e] = 0.0f; }
+  const int g = lane >> 4;
+#pragma unroll
+  for (int t0 = 0; t0 < S::NTH; t0 += S::TD) {
+    if (t0 < nt_out) {
+      __syncthreads();   // LDS free: last weight chunk (first pass) or the previous pass's z read
+#pragma unroll
+      for (int tl = 0; tl < S::TD; ++tl) {
+        if (t0 + tl < nt_out) {
+#pragma unroll
+          for (int pt = 0; pt < PT; ++pt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              zl[(16 * pt + (lane & 15)) * S::ZW + 16 * tl + 4 * g + v] = acc[t0 + tl][pt][v];
+        }
+      }
+      __syncthreads();
+      const int nend = (ns - 16 * t0) < 16 * S::TD ? (ns - 16 * t0) : 16 * S::TD;
+#pragma unroll 2
+      for (int nl = part; nl < nend; nl += LPP) {
+        const int n = 16 * t0 + nl;
+        const float z = zl[p * S::ZW + nl] + bl[n];
+        const float sh = infer_sinh(z);
+        if (m < ng) W[(int64_t)n * ldw + m] = s[n] * (double)sh;
+        // the residual needs ~1e-3: single precision, scaled by s_n (its magnitude only)
+        const float w = (float)s[n] * sh;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float q = (e < ne ? aek[(e < ne ? e : 0) * ns + n] : 0.0f) * w;
+          num[e] += q;
+          den[e] += fabsf(q);
+        }
+      }
+    }
+  }
+  float rr = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    float a = num[e], d = den[e];
+#pragma unroll
+    for (int o = NP; o < 64; o *= 2) {
+      a += __shfl_xor(a, o);
+      d += __shfl_xor(d, o);
+    }
+    // rows past ne are zero: d == 0 gives 0; a non-finite wdot makes q NaN and flags the point
+    const float q = d > 0.0f ? fabsf(a) / d : (d == 0.0f ? 0.0f : d);
+    if (!(q <= rr)) rr = q;
+  }
+  if (part == 0 && m < ng) {
+    resid[m] = rr;
+    flag[m] = (rr <= tol) ? 0 : 1;
+  }
+}
 
 }  // namespace
 
@@ -386,4 +725,55 @@ extern "C" int cg_mlp_scatter(const double* Wi, int64_t ldi, const int64_t* idx,
   scatter_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(Wi, ldi, idx, count, ns, W, ld);
   CG_LAUNCH_CHECK("cg_mlp_scatter launch");
   return CG_OK;
+}
+
+extern "C" int cg_mlp_pack(const void* W, int32_t N, int32_t K, int32_t NT, int32_t KC, void* Wf, void* stream) {
+  if (N <= 0 || K <= 0 || NT * 16 < N || KC * 32 < K) return cg_internal_fail(CG_EINVAL, "cg_mlp_pack: bad shape");
+  const int64_t n = (int64_t)NT * KC * 512;
+  pack_frag_k<<<nblk(n, 256), 256, 0, (hipStream_t)stream>>>((const bf16*)W, N, K, NT, KC, (bf16*)Wf);
+  CG_LAUNCH_CHECK("cg_mlp_pack launch");
+  return CG_OK;
+}
+
+namespace {
+template <int H>
+int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng, int32_t ns,
+                 const float* fscale, const void* Wf, const float* bias, int32_t kc0, int32_t n_hidden,
+                 int32_t nt_out, const double* s, const float* aek, int32_t ne, double tol, double* W,
+                 int64_t ldw, int32_t* flag, double* resid, hipStream_t stream) {
+  using S = InferShape<H>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_k<H>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, S::LDS_BYTES);
+    if (e != hipSuccess) return cg_internal_fail(CG_EHIP, "cg_mlp_infer: cannot reserve LDS");
+    attr_set = true;
+  }
+  const int64_t nb = (ng + S::NW * S::NP - 1) / (S::NW * S::NP);
+  if (nb > 0x7fffffffLL) return cg_internal_fail(CG_ERANGE, "cg_mlp_infer: ng too large");
+  mlp_infer_k<H><<<(unsigned)nb, 64 * S::NW, S::LDS_BYTES, stream>>>(
+      P, T, Y, ld, ng, ns, fscale, (const bf16x8*)Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W, ldw,
+      flag, resid);
+  CG_LAUNCH_CHECK("cg_mlp_infer launch");
+  return CG_OK;
+}
+}  // namespace
+
+extern "C" int cg_mlp_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng,
+                            int32_t ns, const float* fscale, const void* Wf, const float* bias, int32_t hidden,
+                            int32_t kin, int32_t n_hidden, const double* s, const float* aek, int32_t ne,
+                            double tol, double* W, int64_t ldw, int32_t* flag, double* resid, void* stream) {
+  if (ng == 0) return CG_OK;
+  if (ng < 0 || ld < ng || ldw < ng || ns <= 0 || n_hidden < 1 || ne < 0 || ne > 8)
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: bad argument");
+  if (hidden != 256 && hidden != 512)
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: hidden width must be 256 or 512");
+  const int32_t kc0 = (kin + 31) / 32, nt_out = (ns + 15) / 16;
+  if (kin < ns + 2 || kc0 > hidden / 32 || nt_out > hidden / 16)
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: input/output wider than the hidden layers");
+  if (hidden == 256)
+    return launch_infer<256>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
+                             ldw, flag, resid, (hipStream_t)stream);
+  return launch_infer<512>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
+                           ldw, flag, resid, (hipStream_t)stream);
 }

@@ -172,9 +172,15 @@ extern "C" int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n
   for (int32_t b = 0; b < n_patches; ++b) {
     const int64_t a = offsets[b], e = offsets[b + 1];
     if (e < a) return fail(CG_EINVAL, "cg_batched_prepare: offsets must be non-decreasing");
-    for (int64_t s = a; s < e; s += B) {
+    // tiles aligned to multiples of B in the packed arrays (a short head tile first), so that
+    // every full tile reads and writes whole cache lines as the plain launch does
+    int64_t s = a;
+    while (s < e) {
+      int64_t t = (s / B + 1) * B;
+      if (t > e) t = e;
       start.push_back(s);
-      len.push_back((int32_t)((e - s) < B ? (e - s) : B));
+      len.push_back((int32_t)(t - s));
+      s = t;
     }
   }
   const int64_t nt = (int64_t)start.size();
@@ -204,8 +210,10 @@ extern "C" int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const 
   if (!h) return fail(CG_EINVAL, "cg_getrates_batched_prepared: null handle");
   if (h->n_tiles == 0) return CG_OK;
   if (!P || !T || !Y || !W || ld <= 0) return fail(CG_EINVAL, "cg_getrates_batched_prepared: bad array");
-  int64_t g = grid > 0 ? grid : (int64_t)h->n_cu * 8;
+  // grid > 0: that many persistent workgroups; 0: 8 per CU; < 0: one workgroup per tile
+  int64_t g = grid > 0 ? grid : (grid < 0 ? h->n_tiles : (int64_t)h->n_cu * 8);
   if (g > h->n_tiles) g = h->n_tiles;
+  if (g > 0x7fffffffLL) g = 0x7fffffffLL;
   int64_t nt = h->n_tiles;
   void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d,
                   (void*)&h->tile_start_d, (void*)&h->tile_len_d, (void*)&nt, (void*)&ld};

@@ -9,9 +9,10 @@ labelled by the FP64 getrates kernel.
 
 Inference (:meth:`Surrogate.getrates_`) decodes wdot = s sinh(z) in FP64 and computes each
 point's element-conservation residual max_e |sum_k a_ek wdot_k| / sum_k |a_ek wdot_k|; points
-above ``tol`` are packed and recomputed by the FP64 kernel.  An audit recomputes every
-``audit_every``-th point in FP64, replaces it, and if more than ``audit_max_bad`` of the
-audited points are off by more than 1 % the whole batch falls back to FP64.
+above ``tol`` are recomputed by the FP64 kernel.  An audit, run before the network touches
+the rest of the batch, evaluates every ``audit_every``-th point both ways; if more than
+``audit_max_bad`` of them are off by more than 1 % the whole batch goes to FP64, so a poor
+network costs about one FP64 launch plus the audit, not the network plus FP64.
 
 No counterpart exists in the reference (chemgen/chemistry_nn.py:15-85 is a non-functional
 prototype of log-space Arrhenius layers).
@@ -50,6 +51,8 @@ def _bind():
         "cg_mlp_compact": [p, i64, p, p, p],
         "cg_mlp_gather": [p, p, p, i64, p, p, i64, i32, p, p, p, i64, p],
         "cg_mlp_scatter": [p, i64, p, p, i64, i32, p, i64, p],
+        "cg_mlp_pack": [p, i32, i32, i32, i32, p, p],
+        "cg_mlp_infer": [p, p, p, i64, i64, i32, p, p, p, i32, i32, i32, p, p, i32, f64, p, i64, p, p, p],
     }
     for name, a in sig.items():
         f = getattr(L, name)
@@ -89,7 +92,7 @@ class Surrogate:
         _bind()
         self.mech = mech
         self.ns = mech.n_species
-        self.kin = ((self.ns + 2 + 7) // 8) * 8
+        self.kin = ((self.ns + 3 + 7) // 8) * 8   # T, ln P, ln Y_k, 1000/T, zero padding
         self.dims = [self.kin] + [hidden] * n_hidden + [self.ns]
         self.dev = torch.device(device)
         g = torch.Generator(device="cpu").manual_seed(seed)
@@ -99,14 +102,15 @@ class Surrogate:
             w = (torch.randn(N, K, generator=g, dtype=torch.float32) * math.sqrt(2.0 / K)).to(self.dev)
             b = torch.zeros(N, dtype=torch.float32, device=self.dev)
             self.layers.append(Layer(w, b, w.to(torch.bfloat16)))
-        # features: T, ln P and ln(Y + 1e-12), centred and scaled to O(1)
-        self.fscale = torch.tensor([1800.0, 1.0 / 600.0, math.log(1013250.0 * 6.3), 1.0 / 1.1, -5.0, 0.2],
-                                   dtype=torch.float32, device=self.dev)
+        # features: T, ln P, ln(Y + 1e-12) and 1000/T, centred and scaled to O(1)
+        self.fscale = torch.tensor([1800.0, 1.0 / 600.0, math.log(1013250.0 * 6.3), 1.0 / 1.1, -5.0, 0.2,
+                                    0.8, 2.0], dtype=torch.float32, device=self.dev)
         self.s = torch.ones(self.ns, dtype=torch.float64, device=self.dev)
         a = mech.element_matrix()
         a = a[np.abs(a).sum(axis=1) > 0]
         self.aek = torch.tensor(a, dtype=torch.float32, device=self.dev).contiguous()
         self.step_count = 0
+        self._packed = None   # (weights in MFMA operand order, biases) for the fused inference kernel
 
     # ---------------- building blocks ----------------
     def features(self, P, T, Y, stream=None):
@@ -193,6 +197,7 @@ class Surrogate:
     def adam(self, lr: float, b1=0.9, b2=0.999, eps=1e-8, stream=None):
         import torch
         self.step_count += 1
+        self._packed = None
         for L in self.layers:
             if L.mw is None:
                 L.mw, L.vw = torch.zeros_like(L.w), torch.zeros_like(L.w)
@@ -239,56 +244,136 @@ class Surrogate:
         return hist
 
     # ---------------- inference with fallback ----------------
-    def predict_(self, P, T, Y, W, tol: float = 1e-3, stream=None):
-        """Surrogate wdot into W [Ns, ld] (FP64) plus per-point flags/residuals."""
+    @property
+    def fused_ok(self) -> bool:
+        """Whether the one-launch inference kernel covers this network's shape."""
+        hid = self.dims[1:-1]
+        return (len(hid) >= 1 and len(set(hid)) == 1 and hid[0] in (256, 512) and self.kin <= hid[0]
+                and self.ns <= hid[0] and self.aek.shape[0] <= 8)
+
+    def pack(self, stream=None):
+        """Weights in the fused kernel's operand order and the biases back to back.  BF16
+        weights changed directly (not through adam/load_state_dict) need ``_packed = None``."""
+        import torch
+        if self._packed is not None:
+            return self._packed
+        H = self.dims[1]
+        kch, nth = H // 32, H // 16
+        shapes = [(nth, (self.kin + 31) // 32)] + [(nth, kch)] * (len(self.dims) - 3) + [((self.ns + 15) // 16, kch)]
+        Wf = torch.empty(sum(nt * kc * 512 for nt, kc in shapes), dtype=torch.bfloat16, device=self.dev)
+        off = 0
+        for (nt, kc), L, K, N in zip(shapes, self.layers, self.dims[:-1], self.dims[1:]):
+            check(lib().cg_mlp_pack(L.w16.data_ptr(), N, K, nt, kc, Wf[off:].data_ptr(), _s(stream)), "cg_mlp_pack")
+            off += nt * kc * 512
+        bias = torch.cat([L.b for L in self.layers]).contiguous()
+        self._packed = (Wf, bias)
+        return self._packed
+
+    def predict_(self, P, T, Y, W, tol: float = 1e-3, stream=None, fused: Optional[bool] = None):
+        """Surrogate wdot into W [Ns, ld] (FP64) plus per-point flags/residuals.  ``fused``
+        (default: whenever the shape allows) runs the one-launch inference kernel; otherwise
+        the layer-by-layer products of the training path are used."""
         import torch
         ng = P.shape[0]
-        X = self.features(P, T, Y, stream)
-        Zh = self.forward(X, stream=stream)
         flag = torch.empty(ng, dtype=torch.int32, device=self.dev)
         resid = torch.empty(ng, dtype=torch.float64, device=self.dev)
         ld = W.stride(0) if W.shape[0] > 1 else W.shape[1]
+        if fused is None:
+            fused = self.fused_ok
+        if fused:
+            if not self.fused_ok:
+                raise ChemgenError(f"fused inference does not cover network {self.dims}")
+            Wf, bias = self.pack(stream)
+            ldy = Y.stride(0) if Y.shape[0] > 1 else Y.shape[1]
+            check(lib().cg_mlp_infer(P.data_ptr(), T.data_ptr(), Y.data_ptr(), ldy, ng, self.ns,
+                                     self.fscale.data_ptr(), Wf.data_ptr(), bias.data_ptr(), self.dims[1], self.kin,
+                                     len(self.dims) - 2, self.s.data_ptr(), self.aek.data_ptr(), self.aek.shape[0],
+                                     tol, W.data_ptr(), ld, flag.data_ptr(), resid.data_ptr(), _s(stream)),
+                  "cg_mlp_infer")
+            return flag, resid
+        X = self.features(P, T, Y, stream)
+        Zh = self.forward(X, stream=stream)
         check(lib().cg_mlp_decode(Zh.data_ptr(), ng, self.ns, self.s.data_ptr(), self.aek.data_ptr(),
                                   self.aek.shape[0], tol, W.data_ptr(), ld, flag.data_ptr(), resid.data_ptr(),
                                   _s(stream)), "cg_mlp_decode")
         return flag, resid
 
+    def _gather(self, P, T, Y, idx, count, n):
+        """States of the points idx[:n] packed into new [n] / [Ns, n] tensors."""
+        import torch
+        Pp = torch.empty(n, dtype=torch.float64, device=self.dev)
+        Tp = torch.empty_like(Pp)
+        Yp = torch.empty(self.ns, n, dtype=torch.float64, device=self.dev)
+        ldy = Y.stride(0) if Y.shape[0] > 1 else Y.shape[1]
+        check(lib().cg_mlp_gather(P.data_ptr(), T.data_ptr(), Y.data_ptr(), ldy, idx.data_ptr(), count.data_ptr(),
+                                  n, self.ns, Pp.data_ptr(), Tp.data_ptr(), Yp.data_ptr(), n, _s(None)),
+              "cg_mlp_gather")
+        return Pp, Tp, Yp
+
+    def _scatter(self, Wp, idx, count, n, W):
+        ldw = W.stride(0) if W.shape[0] > 1 else W.shape[1]
+        check(lib().cg_mlp_scatter(Wp.data_ptr(), n, idx.data_ptr(), count.data_ptr(), n, self.ns,
+                                   W.data_ptr(), ldw, _s(None)), "cg_mlp_scatter")
+
+    def error(self, Wsur, Wref):
+        """Per-point surrogate error max_k |w_sur - w| / (|w| + s_k) (the accuracy measure)."""
+        return ((Wsur - Wref).abs() / (Wref.abs() + self.s[:, None])).max(dim=0).values
+
     def getrates_(self, P, T, Y, W, fallback: DeviceMechanism, tol: float = 1e-3, audit_every: int = 64,
-                  audit_max_bad: float = 0.05, stream=None) -> dict:
-        """wdot into W via surrogate + per-point FP64 fallback; returns what fell back."""
+                  audit_max_bad: float = 0.05, dense_frac: float = 0.5) -> dict:
+        """wdot into W via surrogate + per-point FP64 fallback; returns what fell back.
+
+        1. Audit first: every ``audit_every``-th point is evaluated by both the surrogate and
+           the FP64 kernel.  If more than ``audit_max_bad`` of them are off by more than 1 %,
+           the whole batch goes to the FP64 kernel and the network is not run on the rest.
+        2. Otherwise the surrogate fills W; the audited points get their FP64 values; every
+           other point whose element-conservation residual exceeds ``tol`` (or is not finite)
+           is recomputed in FP64 -- packed into a dense sub-batch, or, when more than
+           ``dense_frac`` of the points need it, by one FP64 launch over the whole batch.
+        Runs on the current stream (the FP64 launches too) and synchronises with the host
+        twice (audit verdict, fallback count)."""
         import torch
         ng = P.shape[0]
-        flag, resid = self.predict_(P, T, Y, W, tol, stream)
+        info = {"fallback_points": 0, "audit_points": 0, "audit_bad_frac": 0.0, "audit_err_median": None,
+                "audit_err_max": None, "global_fallback": False}
+        if ng == 0:
+            return info
+        aidx = acount = Wa = None
         if audit_every > 0:
-            flag[::audit_every] = 1   # audited points are recomputed in FP64 (and compared)
-            Wsur = W[:, ::audit_every].clone()
+            aidx = torch.arange(0, ng, audit_every, dtype=torch.int64, device=self.dev)
+            na = int(aidx.numel())
+            acount = torch.full((1,), na, dtype=torch.int32, device=self.dev)
+            Pa, Ta, Ya = self._gather(P, T, Y, aidx, acount, na)
+            Wa = torch.empty_like(Ya)
+            fallback.getrates_(Pa, Ta, Ya, Wa)
+            Wsa = torch.empty_like(Ya)
+            self.predict_(Pa, Ta, Ya, Wsa, tol)
+            err = self.error(Wsa, Wa)
+            bad = float((~(err <= 1e-2)).double().mean().item())
+            info.update(audit_points=na, audit_bad_frac=bad, audit_err_median=float(err.median().item()),
+                        audit_err_max=float(err.max().item()))
+            if bad > audit_max_bad:
+                fallback.getrates_(P, T, Y, W)
+                info.update(fallback_points=ng, global_fallback=True)
+                return info
+        flag, _ = self.predict_(P, T, Y, W, tol)
+        if aidx is not None:
+            self._scatter(Wa, aidx, acount, int(aidx.numel()), W)
+            flag[::audit_every] = 0
         count = torch.zeros(1, dtype=torch.int32, device=self.dev)
         idx = torch.empty(ng, dtype=torch.int64, device=self.dev)
-        check(lib().cg_mlp_compact(flag.data_ptr(), ng, count.data_ptr(), idx.data_ptr(), _s(stream)), "cg_mlp_compact")
+        check(lib().cg_mlp_compact(flag.data_ptr(), ng, count.data_ptr(), idx.data_ptr(), _s(None)),
+              "cg_mlp_compact")
         nfb = int(count.item())
-        info = {"fallback_points": nfb, "audit_points": 0, "audit_bad_frac": 0.0, "global_fallback": False}
-        if nfb:
-            Pp = torch.empty(nfb, dtype=torch.float64, device=self.dev)
-            Tp = torch.empty_like(Pp)
-            Yp = torch.empty(self.ns, nfb, dtype=torch.float64, device=self.dev)
+        if nfb > dense_frac * ng:
+            fallback.getrates_(P, T, Y, W)
+            info["fallback_points"] = ng
+        elif nfb:
+            Pp, Tp, Yp = self._gather(P, T, Y, idx, count, nfb)
             Wp = torch.empty_like(Yp)
-            ldy = Y.stride(0) if Y.shape[0] > 1 else Y.shape[1]
-            check(lib().cg_mlp_gather(P.data_ptr(), T.data_ptr(), Y.data_ptr(), ldy, idx.data_ptr(), count.data_ptr(),
-                                      nfb, self.ns, Pp.data_ptr(), Tp.data_ptr(), Yp.data_ptr(), nfb, _s(stream)),
-                  "cg_mlp_gather")
-            fallback.getrates_(Pp, Tp, Yp, Wp, stream)
-            ldw = W.stride(0) if W.shape[0] > 1 else W.shape[1]
-            check(lib().cg_mlp_scatter(Wp.data_ptr(), nfb, idx.data_ptr(), count.data_ptr(), nfb, self.ns,
-                                       W.data_ptr(), ldw, _s(stream)), "cg_mlp_scatter")
-        if audit_every > 0:
-            ref = W[:, ::audit_every]
-            err = ((Wsur - ref).abs() / (ref.abs() + self.s[:, None])).max(dim=0).values
-            bad = float((err > 1e-2).double().mean().item()) if err.numel() else 0.0
-            info["audit_points"] = int(err.numel())
-            info["audit_bad_frac"] = bad
-            if bad > audit_max_bad:
-                fallback.getrates_(P, T, Y, W, stream)
-                info["global_fallback"] = True
+            fallback.getrates_(Pp, Tp, Yp, Wp)
+            self._scatter(Wp, idx, count, nfb, W)
+            info["fallback_points"] = nfb
         return info
 
     def state_dict(self) -> dict:
@@ -302,6 +387,7 @@ class Surrogate:
             L.w.copy_(torch.from_numpy(np.asarray(d[f"w{l}"])))
             L.b.copy_(torch.from_numpy(np.asarray(d[f"b{l}"])))
             L.w16.copy_(L.w.to(torch.bfloat16))
+        self._packed = None
         self.s = torch.from_numpy(np.asarray(d["s"])).to(self.dev).contiguous()
         self.fscale = torch.from_numpy(np.asarray(d["fscale"])).to(self.dev).contiguous()
 

@@ -24,9 +24,9 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
     dm.getrates_(P, T, Y, ref)
     W = torch.empty_like(Y)
     flag, resid = sur.predict_(P, T, Y, W, tol)
-    err_raw = ((W - ref).abs() / (ref.abs() + sur.s[:, None])).max(dim=0).values
+    err_raw = sur.error(W, ref)
     info = sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every)
-    err = ((W - ref).abs() / (ref.abs() + sur.s[:, None])).max(dim=0).values
+    err = sur.error(W, ref)
     torch.cuda.synchronize()
 
     def timed(fn):
@@ -45,6 +45,7 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
     t_sur, wall_sur = timed(lambda: sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every))
     X = sur.features(P, T, Y)
     t_mlp, _ = timed(lambda: sur.forward(X))
+    t_fused, _ = timed(lambda: sur.predict_(P, T, Y, W, tol)) if sur.fused_ok else (float("nan"), 0.0)
     flops = 2.0 * sum(sur.dims[i] * sur.dims[i + 1] for i in range(len(sur.dims) - 1)) * n_eval
     q = lambda e, p: float(torch.quantile(e[: 1 << 20].float(), p).item())
     return {
@@ -61,6 +62,9 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
         "speedup_vs_fp64": t_fp64 / t_sur,
         "mlp_forward_ms": t_mlp * 1e3,
         "mlp_forward_tflops": flops / t_mlp / 1e12,
+        "fused_infer_ms": t_fused * 1e3,
+        "fused_infer_tflops": flops / t_fused / 1e12,
+        "fused_infer_points_per_s": n_eval / t_fused,
     }
 
 

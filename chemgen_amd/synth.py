@@ -36,6 +36,20 @@ def make_states(n_species: int, ng: int, seed: int = 0, T_range=(300.0, 2500.0),
     return P, T, np.ascontiguousarray(Y.T)
 
 
+def ragged_patches(n_patches: int = 64, lo: int = 1 << 8, hi: int = 1 << 17, total: int = 1 << 22,
+                   seed: int = 0) -> np.ndarray:
+    """Patch offsets [n_patches + 1] of a ragged batch: sizes drawn log-uniform in [lo, hi],
+    then rescaled and rounded so that they sum to exactly ``total`` (the remainder of the
+    rounding goes to the largest patches, one point each)."""
+    rng = np.random.default_rng(seed)
+    raw = np.exp(rng.uniform(math.log(lo), math.log(hi), n_patches))
+    sizes = np.floor(raw * (total / raw.sum())).astype(np.int64)
+    rest = int(total - sizes.sum())
+    order = np.argsort(-sizes, kind="stable")
+    sizes[order[:rest]] += 1
+    return np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+
 def _perturb_thermo(th: Nasa7, rng, frac: float) -> Nasa7:
     f = lambda c: tuple(float(x * (1.0 + rng.uniform(-frac, frac))) for x in c)
     return Nasa7(th.t_min, th.t_mid, th.t_max, f(th.low), f(th.high))

@@ -73,7 +73,7 @@ int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_t ng, int64
  * non-decreasing).  The tile list is built on the host and uploaded into a buffer the
  * handle owns (grown on demand), then one persistent kernel walks it; the launch is
  * capturable into a hipGraph once the buffer is large enough.  `grid` = number of
- * persistent workgroups (0 = 8 per CU).  Reference: no counterpart (the reference
+ * persistent workgroups (0 = 8 per CU, negative = one workgroup per tile).  Reference: no counterpart (the reference
  * launches one kernel per array, hip_v1_cpp.j2:339-372). */
 int cg_getrates_batched(cg_mech_t h, const double* P, const double* T, const double* Y,
                         double* wdot, int64_t ld, const int64_t* offsets, int32_t n_patches,
@@ -96,8 +96,8 @@ int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const double* T,
 int cg_mlp_gemm(const void* A, int64_t sAi, int64_t sAk, const void* B, int64_t sBk, int64_t sBj,
                 int32_t M, int32_t N, int32_t K, int32_t ksplit, const float* bias, int32_t act,
                 float* C32, int64_t ldc, void* Cb, int64_t ldcb, int32_t accumulate, void* stream);
-/* X[m, :] = ((T-f0)*f1, (ln P - f2)*f3, (ln(Y_k + 1e-12) - f4)*f5 ..., 0...) in BF16 rows of
- * width kpad (fscale holds f0..f5) */
+/* X[m, :] = ((T-f0)*f1, (ln P - f2)*f3, (ln(Y_k + 1e-12) - f4)*f5 ..., (1000/T - f6)*f7, 0...)
+ * in BF16 rows of width kpad (fscale holds f0..f7) */
 int cg_mlp_features(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng,
                     int32_t ns, int32_t kpad, const float* fscale, void* X, void* stream);
 /* Z[m, k] = asinh(wdot[k, m] / s[k]) */
@@ -127,6 +127,19 @@ int cg_mlp_gather(const double* P, const double* T, const double* Y, int64_t ld,
                   int64_t ldo, void* stream);
 int cg_mlp_scatter(const double* Wi, int64_t ldi, const int64_t* idx, const int32_t* count, int64_t cap,
                    int32_t ns, double* W, int64_t ld, void* stream);
+/* Weights W [N, K] (BF16, row-major) into the operand order of cg_mlp_infer: NT x KC fragments
+ * of 64 lanes x 8 values (NT >= N/16, KC >= K/32; zero padding). */
+int cg_mlp_pack(const void* W, int32_t N, int32_t K, int32_t NT, int32_t KC, void* Wf, void* stream);
+/* Whole-network inference in one launch: features of (P, T, Y[k*ld + m]) -> n_hidden SiLU layers
+ * of width hidden (256 or 512) -> ns outputs z -> wdot W[k*ldw + m] = s[k] sinh(z_k) (FP64) ->
+ * residual/flag as cg_mlp_decode.  Wf: the layers packed by cg_mlp_pack back to back
+ * (NT = hidden/16 for hidden layers, ceil(ns/16) for the output; KC = ceil(kin/32) for the
+ * first layer, hidden/32 after); bias: FP32, hidden values per hidden layer then ns.
+ * Requires ns + 2 <= kin <= hidden, ns <= hidden, ne <= 8. */
+int cg_mlp_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng, int32_t ns,
+                 const float* fscale, const void* Wf, const float* bias, int32_t hidden, int32_t kin,
+                 int32_t n_hidden, const double* s, const float* aek, int32_t ne, double tol, double* W,
+                 int64_t ldw, int32_t* flag, double* resid, void* stream);
 
 #ifdef __cplusplus
 }

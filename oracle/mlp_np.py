@@ -42,6 +42,32 @@ def forward(X, Ws, bs, quantize=True):
     return z, acts, pres
 
 
+def features(P, T, Y, fscale, kin):
+    """Network inputs (before BF16 rounding) of points P[n], T[n], Y[Ns, n]:
+    (T - f0) f1, (ln P - f2) f3, (ln(Y_k + 1e-12) - f4) f5, (1000/T - f6) f7, zero padding to kin,
+    evaluated in FP64 and rounded to FP32 (the GPU evaluates them in FP32, a few FP32 ulps
+    away, before its BF16 conversion)."""
+    f = np.asarray(fscale, dtype=np.float64)
+    ns, n = Y.shape
+    X = np.zeros((n, kin), dtype=np.float64)
+    X[:, 0] = np.float32((T - f[0]) * f[1])
+    X[:, 1] = np.float32((np.log(P) - f[2]) * f[3])
+    X[:, 2:ns + 2] = np.float32((np.log(Y.T + 1e-12) - f[4]) * f[5])
+    if kin > ns + 2:
+        X[:, ns + 2] = np.float32((1000.0 / T - f[6]) * f[7])
+    return X
+
+
+def decode(z, s, aek):
+    """wdot[Ns, n] = s_k sinh(z) and the element-conservation residual per point
+    max_e |sum_k a_ek w_k| / sum_k |a_ek w_k| (0 where the denominator is 0)."""
+    w = np.asarray(s, dtype=np.float64)[:, None] * np.sinh(np.asarray(z, dtype=np.float64).T)
+    t = np.asarray(aek, dtype=np.float64)[:, :, None] * w[None, :, :]
+    num, den = np.abs(t.sum(axis=1)), np.abs(t).sum(axis=1)
+    r = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
+    return w, r.max(axis=0)
+
+
 def loss_and_grads(X, Z, Ws, bs, quantize=True):
     q = bf16 if quantize else (lambda a: np.asarray(a, dtype=np.float64))
     zh, acts, pres = forward(X, Ws, bs, quantize)

@@ -87,8 +87,10 @@ def test_gradients_match_cpu_oracle(torch):
 
 
 def test_fallback_paths(torch):
-    """tol = 0 sends every point to the FP64 kernel (bit-identical output); weights forced
-    to nonsense from the host are caught by the conservation residual."""
+    """tol = 0 sends every point to the FP64 kernel, through the whole-batch launch and
+    through the packed sub-batch (bit-identical output both ways); weights forced to
+    nonsense from the host are caught by the conservation residual, and by the audit
+    alone when the residual test is switched off."""
     from chemgen_amd.surrogate import Surrogate, random_states
     from chemgen_amd.runtime import DeviceMechanism
     m = load_bundled("h2_burke")
@@ -97,19 +99,49 @@ def test_fallback_paths(torch):
     P, T, Y = random_states(m.n_species, 5000, seed=9)
     ref = dm.getrates(P, T, Y)
     sur.fit_scales(ref)
-    W = torch.empty_like(Y)
-    info = sur.getrates_(P, T, Y, W, dm, tol=0.0, audit_every=0)
-    torch.cuda.synchronize()
-    assert info["fallback_points"] == 5000
-    assert torch.equal(W, ref)
+    for dense in (0.5, 1.0):
+        W = torch.full_like(Y, float("nan"))
+        info = sur.getrates_(P, T, Y, W, dm, tol=0.0, audit_every=0, dense_frac=dense)
+        torch.cuda.synchronize()
+        assert info["fallback_points"] == 5000
+        assert torch.equal(W, ref), dense
     for L in sur.layers:       # garbage network: outputs violate element conservation
         L.w16.copy_((L.w * 40.0).to(torch.bfloat16))
-    info = sur.getrates_(P, T, Y, W, dm, tol=1e-3, audit_every=0)
+    W = torch.empty_like(Y)
+    info = sur.getrates_(P, T, Y, W, dm, tol=1e-3, audit_every=0, dense_frac=1.0)
     torch.cuda.synchronize()
-    assert info["fallback_points"] > 4900
-    err = ((W - ref).abs() / (ref.abs() + sur.s[:, None])).max(dim=0).values
-    flagged_ok = float((err <= 1e-12).double().mean())
-    assert flagged_ok >= info["fallback_points"] / 5000 - 1e-9
+    assert info["fallback_points"] > 4900 and not info["global_fallback"]
+    err = sur.error(W, ref)
+    assert float((err <= 1e-12).double().mean()) >= info["fallback_points"] / 5000 - 1e-9
+    W = torch.empty_like(Y)
+    info = sur.getrates_(P, T, Y, W, dm, tol=float("inf"), audit_every=16)
+    torch.cuda.synchronize()
+    print(info)
+    assert info["global_fallback"] and info["audit_points"] == 313 and info["audit_bad_frac"] > 0.5
+    assert torch.equal(W, ref)
+
+
+def test_audit_points_take_fp64_values(torch):
+    """With the residual test off and the audit passing (threshold 1), the audited points
+    hold the FP64 kernel's values and every other point the network's."""
+    from chemgen_amd.surrogate import Surrogate, random_states
+    from chemgen_amd.runtime import DeviceMechanism
+    m = load_bundled("h2_burke")
+    dm = DeviceMechanism(m, build=False)
+    sur = Surrogate(m, hidden=32, n_hidden=2, seed=4)
+    P, T, Y = random_states(m.n_species, 3000, seed=13)
+    ref = dm.getrates(P, T, Y)
+    sur.fit_scales(ref)
+    Wsur = torch.empty_like(Y)
+    sur.predict_(P, T, Y, Wsur, float("inf"))
+    W = torch.empty_like(Y)
+    info = sur.getrates_(P, T, Y, W, dm, tol=float("inf"), audit_every=7, audit_max_bad=1.0)
+    torch.cuda.synchronize()
+    assert info["fallback_points"] == 0 and info["audit_points"] == 429
+    mask = torch.zeros(3000, dtype=torch.bool, device=W.device)
+    mask[::7] = True
+    assert torch.equal(W[:, mask], ref[:, mask])
+    assert torch.equal(W[:, ~mask], Wsur[:, ~mask])
 
 
 def test_training_gains_over_untrained(torch):
@@ -136,3 +168,67 @@ def test_training_gains_over_untrained(torch):
     after = zerr()
     print("z mse before", before, "after", after)
     assert after < 0.25 * before
+
+
+def test_features_match_cpu_oracle(torch):
+    """Network inputs computed on the GPU equal the NumPy restatement after BF16 rounding
+    (a last-bit difference of the FP64 log may move a value by one BF16 step, rarely)."""
+    from chemgen_amd.surrogate import Surrogate, random_states
+    from oracle.mlp_np import features as np_features
+    m = load_bundled("synth_gri53")
+    sur = Surrogate(m, hidden=256, n_hidden=1, seed=0)
+    P, T, Y = random_states(m.n_species, 3000, seed=21)
+    X = sur.features(P, T, Y).float().cpu().numpy().astype(np.float64)
+    ref = bf16(np_features(P.cpu().numpy(), T.cpu().numpy(), Y.cpu().numpy(), sur.fscale.cpu().numpy(), sur.kin))
+    assert X.shape == ref.shape == (3000, 56)
+    diff = np.abs(X - ref)
+    assert float((diff == 0).mean()) >= 0.999
+    assert np.all(diff <= 2.0 ** -7 * np.maximum(np.abs(ref), 1e-30))
+
+
+@pytest.mark.parametrize("name,hidden,layers,ng", [
+    ("h2_burke", 256, 2, 1000), ("synth_gri53", 256, 4, 5000), ("h2_burke2012", 512, 2, 777),
+    ("synth_nhept88", 512, 3, 300),
+])
+def test_fused_inference_matches_oracle(torch, name, hidden, layers, ng):
+    """The one-launch inference kernel against the CPU oracle (features -> BF16 MLP -> decode)
+    on a strided Y, with a point count that leaves a partial tile; and against the
+    layer-by-layer GPU path."""
+    from chemgen_amd.surrogate import Surrogate, random_states
+    from chemgen_amd.runtime import DeviceMechanism
+    from oracle.mlp_np import decode as np_decode, features as np_features, forward as np_forward
+    m = load_bundled(name)
+    dm = DeviceMechanism(m, build=False)
+    sur = Surrogate(m, hidden=hidden, n_hidden=layers, seed=7)
+    for L in sur.layers:          # biases away from zero, so the bias path is exercised
+        L.b.copy_(torch.randn(L.b.shape, device=L.b.device) * 0.3)
+    P, T, Y0 = random_states(m.n_species, ng, seed=31)
+    sur.fit_scales(dm.getrates(P, T, Y0))
+    Y = torch.zeros(m.n_species, ng + 37, dtype=torch.float64, device="cuda:0")[:, :ng]
+    Y.copy_(Y0)
+    W = torch.zeros(m.n_species, ng + 5, dtype=torch.float64, device="cuda:0")[:, :ng]
+    print(f"[fused] {name} H={hidden} L={layers} ng={ng}", flush=True)
+    flag, resid = sur.predict_(P, T, Y, W, tol=1e-3, fused=True)
+    Wl = torch.empty(m.n_species, ng, dtype=torch.float64, device="cuda:0")
+    sur.predict_(P, T, Y, Wl, tol=1e-3, fused=False)
+    torch.cuda.synchronize()
+    s = sur.s.cpu().numpy()
+    Xn = bf16(np_features(P.cpu().numpy(), T.cpu().numpy(), Y0.cpu().numpy(), sur.fscale.cpu().numpy(), sur.kin))
+    Ws = [L.w16.float().cpu().numpy().astype(np.float64) for L in sur.layers]
+    bs = [L.b.cpu().numpy().astype(np.float64) for L in sur.layers]
+    z_ref = np_forward(Xn, Ws, bs, quantize=True)[0]
+    Wn = W.cpu().numpy()
+    z = np.arcsinh(Wn / s[:, None]).T
+    assert np.all(np.isfinite(z))
+    assert np.max(np.abs(z - z_ref) / (1.0 + np.abs(z_ref))) <= 5e-3
+    zl = np.arcsinh(Wl.cpu().numpy() / s[:, None]).T
+    assert np.max(np.abs(z - zl) / (1.0 + np.abs(zl))) <= 5e-3
+    # residual and flag of the kernel's own wdot as the restated decode defines them (the
+    # kernel's sinh and residual sums are single precision: ~1e-7 relative)
+    w_chk, r_chk = np_decode(z, s, sur.aek.cpu().numpy())
+    assert np.max(np.abs(w_chk - Wn) / (np.abs(w_chk) + 1e-7 * s[:, None])) <= 1e-5
+    r = resid.cpu().numpy()
+    assert np.max(np.abs(r - r_chk)) <= 1e-4
+    fl = flag.cpu().numpy()
+    far = np.abs(r_chk - 1e-3) > 2e-4
+    assert np.array_equal(fl[far], (r_chk > 1e-3)[far])
