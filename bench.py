@@ -102,6 +102,49 @@ def bench_batched(steps, warmup):
             "parity": {"sample_points": int(len(pick)), **par}}
 
 
+def bench_large(steps, warmup, ng=1 << 19):
+    """Plain FP64 getrates on the JetSurf-sized mechanism (348 species, 2163 reactions) over
+    2^19 points, T U(600, 2800) K, with the tuned (table-driven) kernel."""
+    import numpy as np
+    import torch
+    from chemgen_amd.autotune import best_variant
+    from chemgen_amd.kernels import load_bundled
+    from chemgen_amd.roofline import PEAK_FP64_VECTOR, work_per_point
+    from chemgen_amd.runtime import DeviceMechanism
+    from chemgen_amd.synth import make_states
+    from oracle.getrates_np import getrates as np_oracle
+    from oracle.metrics import parity_stats
+    name = "synth_jetsurf348"
+    mech = load_bundled(name)
+    variant = best_variant(mech)
+    P, T, Y = make_states(mech.n_species, ng, seed=0, T_range=(600.0, 2800.0))
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
+    Wd = torch.empty_like(Yd)
+    dm = DeviceMechanism(mech, variant, build=False)
+    s = torch.cuda.current_stream()
+    for _ in range(warmup):
+        dm.getrates_(Pd, Td, Yd, Wd, s)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        dm.getrates_(Pd, Td, Yd, Wd, s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / steps / 1e3
+    nchk = 256
+    Wc = Wd[:, :nchk].cpu().numpy()
+    ref, qf, qb = np_oracle(mech, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
+    par = parity_stats(mech, Wc, ref, qf, qb)
+    w = work_per_point(mech)
+    return {"workload": f"{name} ({mech.n_species} species, {mech.n_reactions} reactions), ng={ng}, "
+                        f"T U(600,2800) K, P 1-40 atm log-uniform, Y Dirichlet(0.5), seed 0",
+            "points_per_s": ng / t, "kernel_avg_ms": t * 1e3, "kernel_variant": variant.tag,
+            "flops_per_point": w["flops"], "roofline_frac": w["flops"] * ng / t / PEAK_FP64_VECTOR,
+            "parity": {"sample_points": nchk, **par}}
+
+
 def _finite(o):
     """JSON has no inf/nan: report them as strings."""
     if isinstance(o, dict):
@@ -123,6 +166,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
+    ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized workload")
     ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
                     help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
@@ -215,6 +259,9 @@ def main():
         del Pd, Td, Yd, Wd
         torch.cuda.empty_cache()
         out["batched"] = bench_batched(args.steps, args.warmup)
+    if not args.no_large:
+        torch.cuda.empty_cache()
+        out["large"] = bench_large(max(3, args.steps // 4), min(args.warmup, 2))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
     print(json.dumps(_finite(out)))

@@ -22,14 +22,54 @@ from .mechanism import Mechanism
 TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning.json")
 
 
+LDS_BYTES = 160 * 1024          # per workgroup on gfx950
+STRAIGHT_LINE_MAX_SPECIES = 150  # above this the lane-per-point and split kernels do not fit
+
+
+def table_lds_bytes(ns: int, npt: int, waves: int) -> int:
+    """LDS of one table-kernel tile: EG, h, wdot [ns][npt] and the per-slot partial sums."""
+    return 8 * (3 * ns * npt + waves * 64)
+
+
 def candidate_variants(mech: Mechanism) -> List[Variant]:
-    """Candidates swept by default (all compiled by the build for the bundled mechanisms)."""
-    out = [Variant(64, 1, "lean", "gh", "horner", False),   # lane per point, state in registers
-           Variant(64, 2, "lean", "gh", "horner", True, 4)]  # lane per point, EG in LDS
-    for split, waves in ((4, 2), (4, 3), (6, 3), (8, 3), (8, 4)):   # reaction-split tiles
-        for group in (4, 8):
-            out.append(Variant(64, waves, "lean", "gh", "horner", False, group, split))
+    """Candidates swept by default (all compiled by the build for the bundled mechanisms):
+    straight-line lane-per-point and reaction-split kernels while the species state of a
+    64-point tile fits (registers / LDS), table-driven kernels with every tile length whose
+    LDS fits."""
+    out = []
+    ns = mech.n_species
+    if ns <= STRAIGHT_LINE_MAX_SPECIES:
+        out += [Variant(64, 1, "lean", "gh", "horner", False),   # lane per point, state in registers
+                Variant(64, 2, "lean", "gh", "horner", True, 4)]  # lane per point, EG in LDS
+        for split, waves in ((4, 2), (4, 3), (6, 3), (8, 3), (8, 4)):   # reaction-split tiles
+            if 8 * (2 * ns * 64 + split * 64) <= LDS_BYTES:
+                for group in (4, 8):
+                    out.append(Variant(64, waves, "lean", "gh", "horner", False, group, split))
+    if table_supported(mech):
+        for npt in (16, 32, 64):
+            for waves, wpe in ((4, 1), (8, 2), (16, 4)):
+                if table_lds_bytes(ns, npt, waves) <= LDS_BYTES:
+                    out.append(Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt))
     return out
+
+
+def table_supported(mech: Mechanism) -> bool:
+    """Whether the table kernel's reaction format holds this mechanism (no PLOG, integer
+    stoichiometry up to 3, ...)."""
+    from .codegen import table_records
+    try:
+        table_records(mech)
+    except ValueError:
+        return False
+    return True
+
+
+def default_variant(mech: Mechanism) -> Variant:
+    """Shape used before any tuning: lane per point for small mechanisms, the table kernel
+    (16-point tiles, 8 wavefronts) for large ones."""
+    if mech.n_species <= STRAIGHT_LINE_MAX_SPECIES:
+        return Variant()
+    return Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 16)
 
 
 def _load_db() -> Dict:
@@ -45,7 +85,7 @@ def best_variant(mech: Mechanism) -> Variant:
     e = db.get(mech.name)
     if e and e.get("structure_hash") == f"{structure_hash(mech):016x}":
         return Variant(**e["best"])
-    return Variant()
+    return default_variant(mech)
 
 
 def tune(mech: Mechanism, ng: int = 1 << 20, variants: Optional[List[Variant]] = None,
@@ -94,7 +134,7 @@ def tune(mech: Mechanism, ng: int = 1 << 20, variants: Optional[List[Variant]] =
     good = [r for r in results if r["ok"]]
     best = min(good, key=lambda r: r["ms"]) if good else None
     entry = {"structure_hash": f"{structure_hash(mech):016x}", "ng": ng,
-             "best": best["variant"] if best else dataclasses.asdict(Variant()), "results": results}
+             "best": best["variant"] if best else dataclasses.asdict(default_variant(mech)), "results": results}
     if record and best:
         db = _load_db()
         if out_path and os.path.exists(out_path):

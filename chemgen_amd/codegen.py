@@ -21,7 +21,8 @@ Reaction records (kernel order = mechanism order):
   falloff: the same three for the high- and the low-pressure limit, then
            Troe (1-a, L/T3, a, L/T1[, L T2]) | SRI (a, L b, L/c, d, e) | Lindemann (none),
            then (eff-1) per listed species
-  plog: per pressure point (ln p, p, then A, beta, -Ea per rate) -- read by no kernel yet
+  plog: per pressure point (log2 p, p, 1/(log2 p - log2 p_prev) [0 for the first], then per
+        rate: sign(A), log2|A|, beta, L Ea/Rc), points in increasing pressure
 
 Numbers are taken after the reference's 16-significant-digit printing (``%.15e``,
 chemgen/templates/reactions/standard.j2:5, chemgen/write_constants.py:14-70).
@@ -74,9 +75,13 @@ class Variant:
     group: int = 4           # with LDS state: reactions between re-reads from LDS
     split: int = 1           # wavefronts per 64-point tile sharing the species state in LDS
                              # (each runs 1/split of the reactions); 1 = lane-per-point kernel
+    table: int = 0           # > 0: table-driven kernel, `table` points per tile and `split`
+                             # wavefronts, lanes = (point, reaction slot); for large mechanisms
 
     @property
     def tag(self) -> str:
+        if self.table:
+            return f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
         if self.split > 1:
             return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
                     f"{self.poly[0]}g{self.group}")
@@ -85,11 +90,13 @@ class Variant:
 
     @property
     def threads(self) -> int:
-        return 64 * self.split if self.split > 1 else self.block
+        return 64 * self.split if (self.split > 1 or self.table) else self.block
 
     @property
     def ppb(self) -> int:
         """Grid points per workgroup."""
+        if self.table:
+            return self.table
         return 64 if self.split > 1 else self.block
 
 
@@ -175,11 +182,16 @@ def build_coef(mech: Mechanism):
         elif r.rtype == "plog":
             pts = sorted(set(c[0] for c in r.plog))
             off = len(cb.vals)
+            prev = None
             for p in pts:
                 rates = [c for c in r.plog if c[0] == p]
-                cb.add(r15(math.log(p)), r15(p))
+                lp = math.log2(r15(p))
+                cb.add(lp, r15(p), 0.0 if prev is None else 1.0 / (lp - prev))
+                prev = lp
                 for (_, A, b, Ea) in rates:
-                    cb.add(r15(A), float(b), r15(-Ea))
+                    A = r15(A)
+                    cb.add(1.0 if A >= 0 else -1.0, math.log2(abs(A)) if A != 0 else -1100.0, float(b),
+                           r15(Ea) / RC * LOG2E)
         else:
             raise ValueError(r.rtype)
         if r.third_body:
@@ -212,6 +224,39 @@ def _pf_factor(dnu: float) -> str:
         base = "pinv" if dnu > 0 else "pfac"
         return "(" + " * ".join([base] * int(abs(dnu))) + ")"
     return f"cg_exp2({dlit(-dnu)} * lg2pfac)"
+
+
+def _plog_code(r: Reaction, off: int) -> List[str]:
+    """kf of a PLOG reaction (chemgen/templates/reactions/plog.j2:3-38): log k at each pressure
+    point (the sum of its rates), linear in log P between the two points around P, the end
+    point's value outside the range.  Written without branches: the point values are all
+    evaluated and the interval is picked by selects in increasing pressure."""
+    pts = sorted(set(c[0] for c in r.plog))
+    L = ["    const double lgP = cg_log2(p);"]
+    o = off
+    rec = []
+    for i, pt in enumerate(pts):
+        rates = [c for c in r.plog if c[0] == pt]
+        po = o
+        o += 3
+        if len(rates) == 1:
+            expr = f"__builtin_fma(cf[{o + 2}], lg2T, cf[{o + 1}] - cf[{o + 3}] * invT)"
+            L.append(f"    const double lkp{i} = {expr};")
+            o += 4
+        else:
+            terms = []
+            for _ in rates:
+                terms.append(f"cf[{o}] * cg_exp2(__builtin_fma(cf[{o + 2}], lg2T, cf[{o + 1}] - cf[{o + 3}] * invT))")
+                o += 4
+            L.append(f"    const double lkp{i} = cg_log2({' + '.join(terms)});")
+        rec.append(po)
+    L.append("    double lk = lkp0;")
+    for i in range(1, len(pts)):
+        a, b = rec[i - 1], rec[i]
+        L.append(f"    lk = p >= cf[{a + 1}] ? __builtin_fma(lkp{i} - lkp{i - 1}, (lgP - cf[{a}]) * cf[{b + 2}], lkp{i - 1}) : lk;")
+    L.append(f"    lk = p >= cf[{rec[-1] + 1}] ? lkp{len(pts) - 1} : lk;")
+    L.append("    const double kf = cg_exp2(lk);")
+    return L
 
 
 def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp") -> List[str]:
@@ -248,9 +293,14 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
         nx = {"troe": 4 + (1 if r.troe and len(r.troe) == 4 else 0), "lindemann": 0, "sri": 5}[r.rtype]
         tb_off = off + 6 + nx
     elif r.rtype == "plog":
-        raise NotImplementedError("PLOG reactions are not supported by the HIP target yet")
+        L.extend(_plog_code(r, off))
+    if r.rtype == "plog" and r.reversible and kr_mode != "gh":
+        if kr_mode == "eg":
+            L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
+        else:
+            L.append(f"    const double kr = kf * cg_exp2(-({dg}));")
     M = None
-    if r.rtype != "standard":
+    if r.rtype not in ("standard", "plog"):
         # M = ctot + sum (eff-1) C, in the order the efficiencies are listed
         M = "ctot"
         for q, s in enumerate(r.third_body):
@@ -362,6 +412,8 @@ def partition_reactions(mech: Mechanism, nparts: int) -> List[List[int]]:
 
 def emit_hip(mech: Mechanism, variant: Variant = Variant()):
     """Render the HIP source.  Returns (source, coef, meta dict)."""
+    if variant.table:
+        return _emit_table(mech, variant)
     if variant.split > 1:
         return _emit_split(mech, variant)
     import jinja2
@@ -441,6 +493,98 @@ def _emit_split(mech: Mechanism, variant: Variant):
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h,
                 touched_per_wave=[len(t) for t in touched])
+    return src, coef, meta
+
+
+_TABLE_KIND = {"standard": 0, "third_body": 1, "lindemann": 2, "troe": 3, "sri": 5}
+
+
+def table_order(mech: Mechanism) -> List[int]:
+    """Processing order of the table kernel: reactions grouped by class (kind, reversibility,
+    species counts, efficiency count), so that consecutive reaction slots -- the lane groups
+    of one wavefront -- take the same branches."""
+    def key(j):
+        r = mech.reactions[j]
+        return (_TABLE_KIND[r.rtype], r.reversible, len(r.reactants), len(r.products), len(r.third_body), j)
+    return sorted(range(mech.n_reactions), key=key)
+
+
+def table_records(mech: Mechanism):
+    """(reaction table rows [2 nr][4], efficiency species, order) for the table kernel; raises
+    ValueError for what the table format does not hold (non-integer or > 3 stoichiometric
+    coefficients, > 3 species per side, > 15 efficiencies, > 1023 species, PLOG)."""
+    coef, off_smh, roffs = build_coef(mech)
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    if mech.n_species > 1023:
+        raise ValueError("table kernel: more than 1023 species")
+    for r in mech.reactions:
+        if r.rtype not in _TABLE_KIND:
+            raise ValueError(f"table kernel: reaction type {r.rtype} not supported")
+    rows: List[Tuple[int, int, int, int]] = []
+    effsp: List[int] = []
+    order = table_order(mech)
+    for j in order:
+        r = mech.reactions[j]
+        if r.rtype not in _TABLE_KIND:
+            raise ValueError(f"table kernel: reaction type {r.rtype} not supported")
+        kind = _TABLE_KIND[r.rtype]
+        if r.rtype == "troe" and len(r.troe) == 4:
+            kind = 4
+        if r.rtype == "sri" and len(r.sri) == 5 and r.sri[4] != 0.0:
+            kind = 6
+        if len(r.reactants) > 3 or len(r.products) > 3:
+            raise ValueError("table kernel: more than 3 species on one side")
+        if len(r.third_body) > 15:
+            raise ValueError("table kernel: more than 15 collision efficiencies")
+        if not r.reactants or not r.products:
+            raise ValueError("table kernel: a reaction side is empty")
+        if not float(r.coeff_sum).is_integer() or abs(r.coeff_sum) > 7:
+            raise ValueError(f"table kernel: reaction order change {r.coeff_sum}")
+        x = kind | (int(r.reversible) << 3) | (len(r.reactants) << 4) | (len(r.products) << 6) \
+            | (len(r.third_body) << 8) | ((int(r.coeff_sum) + 8) << 24)
+        z = w = 0
+        for side, base, shift in ((r.reactants, 12, "z"), (r.products, 18, "w")):
+            packed = 0
+            for q, (sp, nu) in enumerate(side.items()):
+                if nu not in (1.0, 2.0, 3.0):
+                    raise ValueError(f"table kernel: stoichiometric coefficient {nu}")
+                x |= (int(nu) - 1) << (base + 2 * q)
+                packed |= idx[sp] << (10 * q)
+            if shift == "z":
+                z = packed
+            else:
+                w = packed
+        nx = {"standard": 0, "third_body": 0, "lindemann": 3, "troe": 3 + (5 if len(r.troe or ()) == 4 else 4),
+              "sri": 3 + 5}[r.rtype]
+        rows.append((x, roffs[j], z, w))
+        rows.append((len(effsp), roffs[j] + 3 + nx, 0, 0))
+        effsp.extend(idx[sp] for sp in r.third_body)
+    return rows, effsp, order
+
+
+def _emit_table(mech: Mechanism, variant: Variant):
+    import jinja2
+    coef, off_smh, roffs = build_coef(mech)
+    rows, effsp, order = table_records(mech)
+    npt, W = variant.table, variant.split
+    if 64 % npt:
+        raise ValueError("table kernel: points per tile must divide 64")
+    slots = W * 64 // npt
+    env = jinja2.Environment(loader=jinja2.FileSystemLoader(TEMPLATE_DIR), keep_trailing_newline=True)
+    tpl = env.get_template("getrates_table_hip.j2")
+    h = structure_hash(mech)
+    rtab = ",\n".join(f"  {{{a}, {b}, {c}, {d}}}" for (a, b, c, d) in rows)
+    src = tpl.render(
+        name=mech.name, ns=mech.n_species, nr=mech.n_reactions, ncoef=len(coef), npt=npt, W=W,
+        threads=variant.threads, ppb=variant.ppb, slots=slots, waves_per_eu=variant.waves_per_eu,
+        hash_hex=f"0x{h:016x}", off_invmw=0, off_smh=off_smh, R0=dlit(R0),
+        lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
+        log10_2=dlit(LOG10_2), log2_10=dlit(LOG2_10), exp_impl=variant.exp_impl, poly=variant.poly,
+        rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
+        tmid=", ".join(dlit(s.thermo.t_mid) for s in mech.species),
+    )
+    meta = dict(ns=mech.n_species, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads,
+                ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, structure_hash=h, slots=slots)
     return src, coef, meta
 
 

@@ -11,6 +11,11 @@ Outputs (Chemkin text, thermo inline):
   synth_gri53.inp       GRI30 species; 325 reactions: 25 Troe + 4 Lindemann, 40 third-body
   synth_nhept88.inp     n-heptane sk88 species; 387 reactions: 33 Troe + 6 Lindemann, 20 third-body
   synth_jetsurf348.inp  JetSurf 2.0 species; 2163 reactions: 150 Troe + 23 SRI, 324 third-body
+  synth_plog53.inp      GRI30 species; 120 reactions: 30 PLOG, 8 Troe, 12 third-body
+
+    python -m chemgen_amd.data.make_synthetic /path/to/reference/ck_files synth_plog53
+
+writes only the named files.
 """
 import os
 import sys
@@ -21,21 +26,26 @@ from ..synth import SPECS, synthetic_mechanism, write_and_parse
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def main(ck_files: str) -> None:
+def main(ck_files: str, only=()) -> None:
     real = {
         "h2_burke": "H2_burke/chem.yaml",
         "h2_burke2012": "Hydrogen_H2/Burke_et_al_2012/chem.yaml",
         "gri30": "GRI30/chem.yaml",
     }
     for name, rel in real.items():
+        if only and name not in only:
+            continue
         m = load_yaml(os.path.join(ck_files, rel), name=name)
         write_chemkin(m, os.path.join(HERE, name + ".inp"))
     bases = {
         "synth_gri53": "GRI30/chem.yaml",
         "synth_nhept88": "n-Heptane_C7H16/llnl_v3.1/nheptane_sk88/chem.yaml",
         "synth_jetsurf348": "JetSurf/2.0/chem.yaml",
+        "synth_plog53": "GRI30/chem.yaml",
     }
     for name, rel in bases.items():
+        if only and name not in only:
+            continue
         base = load_yaml(os.path.join(ck_files, rel))
         m = synthetic_mechanism(base.species, base.elements, name=name, **SPECS[name])
         m2 = write_and_parse(m, os.path.join(HERE, name + ".inp"))
@@ -43,4 +53,4 @@ def main(ck_files: str) -> None:
 
 
 if __name__ == "__main__":
-    main(sys.argv[1])
+    main(sys.argv[1], tuple(sys.argv[2:]))

@@ -17,7 +17,8 @@ Launch shape, the analogue of the reference's parallel levels (--parallel-level,
 --veclen, --nreact-per-block):
   1  one lane per grid point, VECLEN points per workgroup (rounded up to 64)
   2  reaction-split tiles: NREACT_PER_BLOCK wavefronts share a 64-point tile in LDS
-  3  as 2 with 3 waves per SIMD requested from the register allocator
+  3  table-driven tiles of VECLEN points (16, 32 or 64) and NREACT_PER_BLOCK wavefronts, each
+     lane one (point, reaction slot) pair -- the shape for mechanisms too large for 2
   4  the autotuned shape for this mechanism (data/tuning.json), else the default
 
 The reference's python/fortran/OpenMP targets, its Fortran coefficient module and the
@@ -65,9 +66,16 @@ def variant_for(args, mech):
     lvl = args.parallel_level
     if lvl == 1:
         return Variant(block=max(64, (args.veclen + 63) // 64 * 64), waves_per_eu=1)
-    if lvl in (2, 3):
-        return Variant(64, 2 if lvl == 2 else 3, "lean", "gh", "horner", False, 8,
-                       max(2, args.nreact_per_block))
+    if lvl == 2:
+        return Variant(64, 2, "lean", "gh", "horner", False, 8, max(2, args.nreact_per_block))
+    if lvl == 3:
+        from .autotune import LDS_BYTES, table_lds_bytes
+        npt = min((16, 32, 64), key=lambda n: abs(n - args.veclen))
+        waves = max(1, args.nreact_per_block)
+        if table_lds_bytes(mech.n_species, npt, waves) > LDS_BYTES:
+            raise SystemExit(f"--parallel-level 3: {npt} points x {mech.n_species} species exceed the LDS; "
+                             "use a smaller --veclen")
+        return Variant(64, max(1, min(4, waves // 4)), "lean", "gh", "horner", False, 4, waves, npt)
     return best_variant(mech)
 
 

@@ -98,9 +98,11 @@ def _stream_ptr(stream) -> int:
 class DeviceMechanism:
     """One mechanism loaded on the current GPU (code object + gpu_coef blob)."""
 
-    def __init__(self, mech: Mechanism, variant: Variant = Variant(), build: bool = True,
+    def __init__(self, mech: Mechanism, variant: Optional[Variant] = None, build: bool = True,
                  coef: Optional[np.ndarray] = None, code_object: Optional[str] = None):
+        from .autotune import best_variant
         self.mech = mech
+        variant = variant or best_variant(mech)
         self.variant = variant
         self.path = code_object or ensure_code_object(mech, variant, build=build)
         c, _, _ = build_coef(mech)

@@ -58,7 +58,7 @@ def _perturb_thermo(th: Nasa7, rng, frac: float) -> Nasa7:
 def synthetic_mechanism(base_species: Sequence[Species], elements: Sequence[str], n_reactions: int,
                         n_troe: int, n_lindemann: int, n_sri: int, n_third_body: int,
                         seed: int, name: str, frac_irreversible: float = 0.05,
-                        thermo_perturb: float = 0.05, n_small: int = 12) -> Mechanism:
+                        thermo_perturb: float = 0.05, n_small: int = 12, n_plog: int = 0) -> Mechanism:
     rng = np.random.default_rng(seed)
     els = list(elements)
     species = [Species(s.name, dict(s.composition), _perturb_thermo(s.thermo, rng, thermo_perturb))
@@ -129,7 +129,7 @@ def synthetic_mechanism(base_species: Sequence[Species], elements: Sequence[str]
         return (r_idx, p_idx) if d >= 0.0 else (p_idx, r_idx)
 
     kinds = (["troe"] * n_troe + ["lindemann"] * n_lindemann + ["sri"] * n_sri
-             + ["third_body"] * n_third_body)
+             + ["third_body"] * n_third_body + ["plog"] * n_plog)
     kinds += ["standard"] * (n_reactions - len(kinds))
     order = rng.permutation(len(kinds))
     reactions: List[Reaction] = []
@@ -139,6 +139,19 @@ def synthetic_mechanism(base_species: Sequence[Species], elements: Sequence[str]
         if kind == "standard":
             r_idx, p_idx = orient(*(bimol() if rng.uniform() < 0.85 else assoc()))
             r = Reaction("", st(r_idx), st(p_idx), "standard", rev, arrh())
+        elif kind == "plog":
+            # 3-6 pressure points log-uniform in 0.05-100 atm, one or two Arrhenius terms each
+            # (a PLOG sum), drawn around one base rate so that k(P) varies smoothly
+            r_idx, p_idx = orient(*bimol())
+            A, b, Ea = arrh()
+            pts = np.sort(10.0 ** rng.uniform(math.log10(0.05), 2.0, int(rng.integers(3, 7)))) * 1013250.0
+            plog = []
+            for pp in pts:
+                nr = 1 if rng.uniform() < 0.7 else 2
+                for _ in range(nr):
+                    plog.append((float(pp), A * 10.0 ** rng.uniform(-1.0, 1.0) / nr,
+                                 b + float(rng.uniform(-0.3, 0.3)), Ea * float(rng.uniform(0.8, 1.2))))
+            r = Reaction("", st(r_idx), st(p_idx), "plog", rev, plog[0][1:], plog=plog)
         elif kind == "third_body":
             r_idx, p_idx = orient(*assoc())
             r = Reaction("", st(r_idx), st(p_idx), "third_body", rev, arrh())
@@ -176,4 +189,6 @@ SPECS = {
     "synth_nhept88": dict(n_reactions=387, n_troe=33, n_lindemann=6, n_sri=0, n_third_body=20, seed=88),
     "synth_jetsurf348": dict(n_reactions=2163, n_troe=150, n_lindemann=0, n_sri=23, n_third_body=324,
                              seed=348),
+    "synth_plog53": dict(n_reactions=120, n_troe=8, n_lindemann=0, n_sri=0, n_third_body=12, n_plog=30,
+                         seed=531),
 }

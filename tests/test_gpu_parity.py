@@ -35,6 +35,8 @@ def _run(torch, mech, P, T, Y, dm=None):
     ("gri30", 2048, (300.0, 2500.0)),
     ("synth_gri53", 2048, (300.0, 2500.0)),
     ("synth_nhept88", 1024, (600.0, 2800.0)),
+    ("synth_jetsurf348", 512, (600.0, 2800.0)),
+    ("synth_plog53", 2048, (300.0, 2500.0)),
 ])
 def test_parity_vs_oracle(torch_cuda, name, ng, trange):
     m = load_bundled(name)
@@ -56,3 +58,22 @@ def test_ragged_tail_sizes(torch_cuda, ng):
     w = _run(torch_cuda, m, P, T, Y)
     ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
     assert parity_stats(m, w, ref, qf, qb)["max_gross"] <= TOL_GROSS
+
+
+@pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348"])
+def test_table_kernels_vs_oracle(torch_cuda, name):
+    """Every table-driven candidate (all tile lengths and wavefront counts that fit the LDS)
+    against the oracle; gri30 brings Troe with and without T2 and Lindemann reactions, the
+    JetSurf-sized mechanism SRI; 300 points leave a partial tile for every tile length."""
+    from chemgen_amd.autotune import candidate_variants
+    from chemgen_amd.runtime import DeviceMechanism
+    m = load_bundled(name)
+    P, T, Y = make_states(m.n_species, 300, seed=4, T_range=(300.0, 2500.0))
+    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    tables = [v for v in candidate_variants(m) if v.table]
+    assert tables
+    for v in tables:
+        print(f"[table] {name} {v.tag}", flush=True)
+        w = _run(torch_cuda, m, P, T, Y, DeviceMechanism(m, v, build=False))
+        st = parity_stats(m, w, ref, qf, qb)
+        assert st["finite"] and st["max_gross"] <= TOL_GROSS, (v.tag, st)

@@ -211,6 +211,38 @@ def test_sri_at_unit_reduced_pressure():
     assert -getrates(m, P, Tt, Y)[0, 0] == pytest.approx(A * 0.5 * F * C[0] * C[1], rel=1e-13)
 
 
+def test_plog_interpolation_closed_form():
+    """PLOG (reactions/plog.j2:3-38): the rate at a listed pressure is that point's sum of
+    Arrhenius terms, between two points ln k is linear in ln P, outside the range the end
+    point's rate holds."""
+    atm = PATM
+    T = 1300.0
+    pts = [(1.0, [(2.0e12, 0.5, 8000.0)]), (10.0, [(3.0e12, 0.2, 6000.0), (4.0e11, 1.0, 2000.0)]),
+           (100.0, [(9.0e12, 0.0, 5000.0)])]
+    plog = [(p * atm, A, b, Ea) for p, rates in pts for (A, b, Ea) in rates]
+    r = _irrev({"H": 1.0, "O2": 1.0}, {"HO2": 1.0}, plog[0][1:], rtype="plog", plog=plog)
+    m = _mech(["H", "O2", "HO2", "N2"], [r])
+
+    def k_at(p_atm):
+        X = np.array([0.2, 0.3, 0.0, 0.5])
+        C = X * p_atm * atm / (R0 * T)
+        P, Tt, Y = _state(m, C, T)
+        return -getrates(m, P, Tt, Y)[0, 0] / (C[0] * C[1])
+
+    def ksum(rates):
+        return sum(A * T ** b * math.exp(-Ea / (RC * T)) for (A, b, Ea) in rates)
+
+    k = [ksum(rates) for _, rates in pts]
+    assert k_at(0.3) == pytest.approx(k[0], rel=1e-12)
+    assert k_at(10.0) == pytest.approx(k[1], rel=1e-12)
+    assert k_at(100.0) == pytest.approx(k[2], rel=1e-12)
+    assert k_at(5000.0) == pytest.approx(k[2], rel=1e-12)
+    assert math.log(k_at(math.sqrt(10.0))) == pytest.approx(0.5 * (math.log(k[0]) + math.log(k[1])), rel=1e-12)
+    w = 0.3  # ln P a fraction w of the way from 10 to 100 atm
+    assert math.log(k_at(10.0 ** (1.0 + w))) == pytest.approx((1 - w) * math.log(k[1]) + w * math.log(k[2]),
+                                                              rel=1e-12)
+
+
 def test_pressure_scaling():
     r1 = _irrev({"H": 1.0, "O2": 1.0}, {"O": 1.0, "OH": 1.0}, (1e13, 0.5, 3000.0))
     r2 = _irrev({"H": 1.0, "OH": 1.0}, {"H2O": 1.0}, (1e16, -1.0, 0.0), rtype="third_body")
