@@ -509,14 +509,18 @@ def table_order(mech: Mechanism) -> List[int]:
     return sorted(range(mech.n_reactions), key=key)
 
 
-def table_records(mech: Mechanism):
-    """(reaction table rows [2 nr][4], efficiency species, order) for the table kernel; raises
-    ValueError for what the table format does not hold (non-integer or > 3 stoichiometric
-    coefficients, > 3 species per side, > 15 efficiencies, > 1023 species, PLOG)."""
+def table_records(mech: Mechanism, npt: int = 16):
+    """(reaction table rows [2 nr][4], efficiency species, order) for the table kernel with
+    ``npt``-point tiles.  Each side of a reaction is written as up to three species *slots*
+    (a species with coefficient nu fills nu slots), absent slots point at an extra LDS row
+    that holds EG = h = 1; slots are stored as LDS row offsets (species index x npt).
+    Raises ValueError for what the format does not hold (more than 3 slots per side,
+    non-integer coefficients, more than 15 efficiencies, PLOG)."""
     coef, off_smh, roffs = build_coef(mech)
     idx = {s.name: i for i, s in enumerate(mech.species)}
-    if mech.n_species > 1023:
-        raise ValueError("table kernel: more than 1023 species")
+    pad = mech.n_species
+    if (pad + 1) * npt >= 1 << 16:
+        raise ValueError("table kernel: LDS row offsets exceed 16 bits")
     for r in mech.reactions:
         if r.rtype not in _TABLE_KIND:
             raise ValueError(f"table kernel: reaction type {r.rtype} not supported")
@@ -525,39 +529,33 @@ def table_records(mech: Mechanism):
     order = table_order(mech)
     for j in order:
         r = mech.reactions[j]
-        if r.rtype not in _TABLE_KIND:
-            raise ValueError(f"table kernel: reaction type {r.rtype} not supported")
         kind = _TABLE_KIND[r.rtype]
         if r.rtype == "troe" and len(r.troe) == 4:
             kind = 4
         if r.rtype == "sri" and len(r.sri) == 5 and r.sri[4] != 0.0:
             kind = 6
-        if len(r.reactants) > 3 or len(r.products) > 3:
-            raise ValueError("table kernel: more than 3 species on one side")
         if len(r.third_body) > 15:
             raise ValueError("table kernel: more than 15 collision efficiencies")
-        if not r.reactants or not r.products:
-            raise ValueError("table kernel: a reaction side is empty")
-        if not float(r.coeff_sum).is_integer() or abs(r.coeff_sum) > 7:
-            raise ValueError(f"table kernel: reaction order change {r.coeff_sum}")
-        x = kind | (int(r.reversible) << 3) | (len(r.reactants) << 4) | (len(r.products) << 6) \
-            | (len(r.third_body) << 8) | ((int(r.coeff_sum) + 8) << 24)
-        z = w = 0
-        for side, base, shift in ((r.reactants, 12, "z"), (r.products, 18, "w")):
-            packed = 0
-            for q, (sp, nu) in enumerate(side.items()):
-                if nu not in (1.0, 2.0, 3.0):
+        slots = []
+        for side in (r.reactants, r.products):
+            sl = []
+            for sp, nu in side.items():
+                if not float(nu).is_integer():
                     raise ValueError(f"table kernel: stoichiometric coefficient {nu}")
-                x |= (int(nu) - 1) << (base + 2 * q)
-                packed |= idx[sp] << (10 * q)
-            if shift == "z":
-                z = packed
-            else:
-                w = packed
+                sl += [idx[sp]] * int(nu)
+            if not 1 <= len(sl) <= 3:
+                raise ValueError("table kernel: a reaction side needs 1 to 3 species slots")
+            slots.append(sl)
+        nrs, nps = len(slots[0]), len(slots[1])
+        dnu = nps - nrs
+        x = kind | (int(r.reversible) << 3) | (nrs << 4) | (nps << 6) | (len(r.third_body) << 8) \
+            | ((dnu + 8) << 24)
+        rr = [k * npt for k in slots[0]] + [pad * npt] * (3 - nrs)
+        pp = [k * npt for k in slots[1]] + [pad * npt] * (3 - nps)
         nx = {"standard": 0, "third_body": 0, "lindemann": 3, "troe": 3 + (5 if len(r.troe or ()) == 4 else 4),
               "sri": 3 + 5}[r.rtype]
-        rows.append((x, roffs[j], z, w))
-        rows.append((len(effsp), roffs[j] + 3 + nx, 0, 0))
+        rows.append((x, roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16)))
+        rows.append((pp[1] | (pp[2] << 16), len(effsp), roffs[j] + 3 + nx, 0))
         effsp.extend(idx[sp] for sp in r.third_body)
     return rows, effsp, order
 
@@ -565,8 +563,8 @@ def table_records(mech: Mechanism):
 def _emit_table(mech: Mechanism, variant: Variant):
     import jinja2
     coef, off_smh, roffs = build_coef(mech)
-    rows, effsp, order = table_records(mech)
     npt, W = variant.table, variant.split
+    rows, effsp, order = table_records(mech, npt)
     if 64 % npt:
         raise ValueError("table kernel: points per tile must divide 64")
     slots = W * 64 // npt

@@ -21,6 +21,11 @@ Launch shape, the analogue of the reference's parallel levels (--parallel-level,
      lane one (point, reaction slot) pair -- the shape for mechanisms too large for 2
   4  the autotuned shape for this mechanism (data/tuning.json), else the default
 
+``--legacy-abi`` (with ``--compat reference``) also writes ``<stem>_legacy.so``: the
+reference's C entry points for this mechanism (copyConstantsToDevice taking the reference's
+coefficient arrays, getrates_hip_wrapper_v1..v4, allocate_intermediate_DeviceMemory) over the
+generated kernel, so that the reference's Fortran module links it unchanged.
+
 The reference's python/fortran/OpenMP targets, its Fortran coefficient module and the
 mini-app are not produced here (those targets exist in the reference).
 """
@@ -57,6 +62,9 @@ def parse_arguments(argv=None):
     p.add_argument("--time-cpu", action="store_true")
     p.add_argument("--compat", choices=["full", "reference"], default="full",
                    help="reference: keep only the reaction types the reference handles")
+    p.add_argument("--legacy-abi", action="store_true",
+                   help="also build lib<stem>_legacy.so exporting the reference's C entry points "
+                        "(copyConstantsToDevice, getrates_hip_wrapper_v1..v4) over this kernel")
     return p.parse_args(argv)
 
 
@@ -128,6 +136,10 @@ def getrates(P, T, Y):
     """wdot = getrates(P, T, Y): P [dyn/cm^2], T [K], Y mass fractions -> wdot [mol/cm^3/s]."""
     return _impl(P, T, Y)
 ''')
+    if args.legacy_abi:
+        from .legacy_abi import build_legacy_library
+        build_legacy_library(mech, os.path.abspath(stem + ".hsaco"), stem + "_legacy.so")
+        print(f"reference C ABI library written to {stem}_legacy.so")
     print(f"Expressions and getrates function written to {args.output}")
 
 
