@@ -145,6 +145,69 @@ def bench_large(steps, warmup, ng=1 << 19):
             "parity": {"sample_points": nchk, **par}}
 
 
+def bench_surrogate(steps, warmup, ng=1 << 20):
+    """The learned surrogate for the GRI-sized mechanism (4 x 256 SiLU MLP, BF16 on the matrix
+    cores, weights in chemgen_amd/data) on 2^20 held-out states (T U(800, 2800) K, seed 7):
+    its error against the FP64 kernel per point, max_k |w_sur - w| / (|w| + s_k) with s_k = 1e-3
+    of the 99th percentile of |w_k| over the training labels, before and after the per-point
+    FP64 fallback, and the throughput of the fused inference kernel and of surrogate +
+    fallback (timed on the current stream with HIP events)."""
+    import numpy as np
+    import torch
+    from chemgen_amd.autotune import best_variant
+    from chemgen_amd.kernels import load_bundled
+    from chemgen_amd.roofline import PEAK_BF16_DENSE
+    from chemgen_amd.runtime import DeviceMechanism
+    from chemgen_amd.surrogate import Surrogate, random_states
+    name = "synth_gri53"
+    mech = load_bundled(name)
+    dm = DeviceMechanism(mech, best_variant(mech), build=False)
+    sur = Surrogate(mech, hidden=256, n_hidden=4, seed=0)
+    sur.load_state_dict(dict(np.load(os.path.join(ROOT, "chemgen_amd", "data", "surrogate_synth_gri53.npz"))))
+    P, T, Y = random_states(mech.n_species, ng, seed=7)
+    ref = torch.empty_like(Y)
+    dm.getrates_(P, T, Y, ref)
+    W = torch.empty_like(Y)
+    flag, _ = sur.predict_(P, T, Y, W)
+    err_sur = sur.error(W, ref)
+    info = sur.getrates_(P, T, Y, W, dm)
+    err = sur.error(W, ref)
+    s = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(steps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps / 1e3
+
+    t_fp64 = timed(lambda: dm.getrates_(P, T, Y, ref, s))
+    t_inf = timed(lambda: sur.predict_(P, T, Y, W))
+    t_path = timed(lambda: sur.getrates_(P, T, Y, W, dm))
+    flops = 2.0 * sum(sur.dims[i] * sur.dims[i + 1] for i in range(len(sur.dims) - 1))
+    q = lambda e, p: float(torch.quantile(e.float(), p).item())
+    return {"workload": f"{name}: MLP {sur.dims}, SiLU, BF16 MFMA / FP32 accumulation, {ng} held-out states "
+                        f"(T U(800,2800) K, P 1-40 atm, Y Dirichlet(0.5), seed 7); fallback: element residual "
+                        f"> 1e-3 or audit (every 64th point) > 5% off by > 1%",
+            "fp64_points_per_s": ng / t_fp64,
+            "surrogate_only": {"frac_err_le_1pct": float((err_sur <= 1e-2).double().mean()),
+                               "median_err": q(err_sur, 0.5), "p99_err": q(err_sur, 0.99),
+                               "residual_flag_frac": float(flag.double().mean())},
+            "with_fallback": {"frac_err_le_1pct": float((err <= 1e-2).double().mean()),
+                              "fallback_frac": info["fallback_points"] / ng,
+                              "global_fallback": info["global_fallback"]},
+            "inference_points_per_s": ng / t_inf,
+            "inference_roofline": {"bound": "mfma", "achieved": flops * ng / t_inf / 1e12,
+                                   "peak": PEAK_BF16_DENSE / 1e12, "unit": "TFLOP/s",
+                                   "frac": flops * ng / t_inf / PEAK_BF16_DENSE, "flops_per_point": flops},
+            "path_points_per_s": ng / t_path, "path_speedup_vs_fp64": t_fp64 / t_path}
+
+
 def _finite(o):
     """JSON has no inf/nan: report them as strings."""
     if isinstance(o, dict):
@@ -167,6 +230,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
     ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized workload")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate workload")
     ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
                     help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
@@ -262,6 +326,9 @@ def main():
     if not args.no_large:
         torch.cuda.empty_cache()
         out["large"] = bench_large(max(3, args.steps // 4), min(args.warmup, 2))
+    if not args.no_surrogate:
+        torch.cuda.empty_cache()
+        out["surrogate"] = bench_surrogate(args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
     print(json.dumps(_finite(out)))

@@ -392,13 +392,19 @@ class Surrogate:
         self.fscale = torch.from_numpy(np.asarray(d["fscale"])).to(self.dev).contiguous()
 
 
-def random_states(ns: int, n: int, seed: int, T_range=(800.0, 2800.0), P_atm=(1.0, 40.0), device="cuda:0"):
-    """Seeded states drawn on the GPU (T uniform, P log-uniform, Y Dirichlet(0.5))."""
+def random_states(ns: int, n: int, seed: int, T_range=(800.0, 2800.0), P_atm=(1.0, 40.0), device="cuda:0",
+                  chunk: int = 1 << 21):
+    """Seeded states drawn on the GPU (T uniform, P log-uniform, Y Dirichlet(0.5)); Y is drawn
+    ``chunk`` points at a time, so the temporaries stay small next to Y itself."""
     import torch
     g = torch.Generator(device=device).manual_seed(seed)
     T = torch.rand(n, generator=g, device=device, dtype=torch.float64) * (T_range[1] - T_range[0]) + T_range[0]
     lp = torch.rand(n, generator=g, device=device, dtype=torch.float64) * math.log(P_atm[1] / P_atm[0])
     P = 1013250.0 * P_atm[0] * torch.exp(lp)
-    gam = torch._standard_gamma(torch.full((ns, n), 0.5, dtype=torch.float64, device=device), generator=g)
-    Y = gam / gam.sum(dim=0, keepdim=True)
-    return P.contiguous(), T.contiguous(), Y.contiguous()
+    Y = torch.empty(ns, n, dtype=torch.float64, device=device)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        gam = torch._standard_gamma(torch.full((ns, b - a), 0.5, dtype=torch.float64, device=device), generator=g)
+        Y[:, a:b] = gam / gam.sum(dim=0, keepdim=True)
+        del gam
+    return P.contiguous(), T.contiguous(), Y

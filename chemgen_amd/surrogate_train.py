@@ -17,11 +17,19 @@ import time
 
 
 def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: int = 64, reps: int = 5):
+    """Accuracy (before / after the fallback) and throughput on n_eval held-out states."""
     import torch
     from .surrogate import random_states
     P, T, Y = random_states(sur.ns, n_eval, seed=seed)
     ref = torch.empty_like(Y)
     dm.getrates_(P, T, Y, ref)
+    # the FP64 kernel against itself: zero for the deterministic kernels; the table kernel's
+    # LDS atomics reorder additions between runs, which shows where a species' net rate is a
+    # near-cancellation of much larger terms
+    rep = torch.empty_like(Y)
+    dm.getrates_(P, T, Y, rep)
+    err_rep = sur.error(rep, ref)
+    del rep
     W = torch.empty_like(Y)
     flag, resid = sur.predict_(P, T, Y, W, tol)
     err_raw = sur.error(W, ref)
@@ -56,6 +64,8 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
                           "p99_err": q(err, 0.99), **info,
                           "fallback_frac": info["fallback_points"] / n_eval},
         "residual_flag_frac": float(flag.double().mean()),
+        "fp64_repeat": {"frac_le_1pct": float((err_rep <= 1e-2).double().mean()),
+                        "median_err": q(err_rep, 0.5), "max_err": float(err_rep.max().item())},
         "fp64_kernel_points_per_s": n_eval / t_fp64,
         "surrogate_path_points_per_s": n_eval / t_sur,
         "surrogate_path_wall_points_per_s": n_eval / wall_sur,
