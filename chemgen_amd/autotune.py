@@ -46,6 +46,9 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
             if 8 * (2 * ns * 64 + split * 64) <= LDS_BYTES:
                 for group in (4, 8):
                     out.append(Variant(64, waves, "lean", "gh", "horner", False, group, split))
+                # the same with the coefficients compiled in
+                out.append(Variant(64, waves, "lean", "gh", "horner", False, 8, split, 0, True))
+        out.append(Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, True))
     if table_supported(mech):
         for npt in (16, 32, 64):
             for waves, wpe in ((4, 1), (8, 2), (16, 4)):

@@ -77,16 +77,19 @@ class Variant:
                              # (each runs 1/split of the reactions); 1 = lane-per-point kernel
     table: int = 0           # > 0: table-driven kernel, `table` points per tile and `split`
                              # wavefronts, lanes = (point, reaction slot); for large mechanisms
+    baked: bool = False      # compile the coefficient values into the code (no scalar loads that
+                             # share the LGKM counter with LDS reads); the blob must then match
 
     @property
     def tag(self) -> str:
         if self.table:
             return f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
+        c = "k" if self.baked else ""
         if self.split > 1:
             return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
-                    f"{self.poly[0]}g{self.group}")
+                    f"{self.poly[0]}g{self.group}{c}")
         return (f"b{self.block}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
-                f"{self.poly[0]}{'L%d' % self.group if self.g_lds else ''}")
+                f"{self.poly[0]}{'L%d' % self.group if self.g_lds else ''}{c}")
 
     @property
     def threads(self) -> int:
@@ -410,6 +413,22 @@ def partition_reactions(mech: Mechanism, nparts: int) -> List[List[int]]:
     return [sorted(p) for p in parts]
 
 
+def coef_hash(coef: np.ndarray) -> int:
+    """FNV-1a (64 bit) over the bytes of the FP64 blob, as cg_mech_load computes it."""
+    h = 0xCBF29CE484222325
+    for b in np.ascontiguousarray(coef, dtype=np.float64).tobytes():
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _baked_args(coef: np.ndarray, variant: "Variant") -> dict:
+    if not variant.baked:
+        return {"baked": False}
+    vals = ",\n".join(", ".join(repr(float(x)) if math.isfinite(x) else ("__builtin_inf()" if x > 0 else "-__builtin_inf()")
+                                for x in coef[i:i + 4]) for i in range(0, len(coef), 4))
+    return {"baked": True, "baked_values": vals, "coef_hash_hex": f"0x{coef_hash(coef):016x}"}
+
+
 def emit_hip(mech: Mechanism, variant: Variant = Variant()):
     """Render the HIP source.  Returns (source, coef, meta dict)."""
     if variant.table:
@@ -447,6 +466,7 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
         g_lds=variant.g_lds and variant.kr_mode in ("eg", "gh"),
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of,
         reactions_code="\n".join(lines), threads=variant.threads, ppb=variant.ppb,
+        **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h)
@@ -488,7 +508,7 @@ def _emit_split(mech: Mechanism, variant: Variant):
         off_smh=off_smh, R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0),
         r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of, species_of=species_of,
-        zero_of=species_of, touched=touched, reactions_code=codes,
+        zero_of=species_of, touched=touched, reactions_code=codes, **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h,

@@ -37,7 +37,16 @@ int fail(int code, const char* fmt, ...) {
 struct Meta {
   int32_t abi, ns, nr, ncoef, block, ppb;  // block = threads per workgroup, ppb = points per workgroup
   uint64_t structure_hash;
+  uint64_t coef_hash;  // non-zero: the code object has the blob compiled in; it must match
 };
+
+// FNV-1a over the bytes of the blob (the generator computes the same for compiled-in blobs)
+uint64_t coef_fnv(const double* c, int64_t n) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(c);
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (int64_t i = 0; i < n * (int64_t)sizeof(double); ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  return h;
+}
 
 }  // namespace
 
@@ -92,11 +101,12 @@ extern "C" int cg_mech_load(const char* path, const double* coef, int64_t n_coef
     delete h;
     return fail(CG_ELOAD, "code object %s: %s", path, hipGetErrorString(e));
   }
-  if (h->meta.abi != 2 || h->meta.ppb <= 0 || h->meta.ncoef != n_coef ||
+  if (h->meta.abi != 3 || h->meta.ppb <= 0 || h->meta.ncoef != n_coef ||
+      (h->meta.coef_hash != 0 && h->meta.coef_hash != coef_fnv(coef, n_coef)) ||
       (expect_hash != 0 && h->meta.structure_hash != expect_hash)) {
     int code = fail(CG_ELOAD,
                     "code object %s does not match the coefficients (abi %d, ncoef %d vs %lld, "
-                    "hash %016llx vs %016llx)",
+                    "hash %016llx vs %016llx, or a compiled-in blob that differs)",
                     path, h->meta.abi, h->meta.ncoef, (long long)n_coef,
                     (unsigned long long)h->meta.structure_hash, (unsigned long long)expect_hash);
     (void)hipModuleUnload(h->mod);
@@ -123,6 +133,8 @@ extern "C" int cg_mech_load(const char* path, const double* coef, int64_t n_coef
 extern "C" int cg_mech_set_coef(cg_mech_t h, const double* coef, int64_t n_coef) {
   if (!h || !coef) return fail(CG_EINVAL, "cg_mech_set_coef: bad argument");
   if (n_coef != h->meta.ncoef) return fail(CG_EINVAL, "cg_mech_set_coef: expected %d coefficients", h->meta.ncoef);
+  if (h->meta.coef_hash != 0 && h->meta.coef_hash != coef_fnv(coef, n_coef))
+    return fail(CG_EINVAL, "cg_mech_set_coef: this code object has its coefficients compiled in");
   CG_HIP(hipMemcpy(h->coef_d, coef, sizeof(double) * n_coef, hipMemcpyHostToDevice));
   return CG_OK;
 }

@@ -67,6 +67,9 @@ def check(rc: int, what: str) -> None:
 
 def code_object_path(mech: Mechanism, variant: Variant) -> str:
     h = structure_hash(mech)
+    if variant.baked:   # the coefficient values are part of the code: name them too
+        from .codegen import build_coef, coef_hash
+        h ^= coef_hash(build_coef(mech)[0])
     return os.path.join(KERNEL_DIR, f"{mech.name}_{h:016x}_{variant.tag}.hsaco")
 
 

@@ -77,3 +77,24 @@ def test_table_kernels_vs_oracle(torch_cuda, name):
         w = _run(torch_cuda, m, P, T, Y, DeviceMechanism(m, v, build=False))
         st = parity_stats(m, w, ref, qf, qb)
         assert st["finite"] and st["max_gross"] <= TOL_GROSS, (v.tag, st)
+
+
+def test_baked_coefficients_vs_oracle_and_guard(torch_cuda):
+    """A kernel with the coefficients compiled in: parity with the oracle, and a blob that
+    differs from the compiled one is refused at load and at cg_mech_set_coef."""
+    from chemgen_amd.codegen import Variant, build_coef
+    from chemgen_amd.runtime import ChemgenError, DeviceMechanism
+    m = load_bundled("synth_gri53")
+    v = Variant(64, 2, "lean", "gh", "horner", False, 8, 4, 0, True)
+    P, T, Y = make_states(m.n_species, 2048, seed=6)
+    dm = DeviceMechanism(m, v, build=False)
+    w = _run(torch_cuda, m, P, T, Y, dm)
+    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    st = parity_stats(m, w, ref, qf, qb)
+    assert st["finite"] and st["max_gross"] <= TOL_GROSS, st
+    coef = build_coef(m)[0].copy()
+    coef[-1] *= 1.5
+    with pytest.raises(ChemgenError):
+        dm.set_coef(coef)
+    with pytest.raises(ChemgenError):
+        DeviceMechanism(m, v, build=False, coef=coef)
