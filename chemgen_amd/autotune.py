@@ -42,13 +42,19 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
     if ns <= STRAIGHT_LINE_MAX_SPECIES:
         out += [Variant(64, 1, "lean", "gh", "horner", False),   # lane per point, state in registers
                 Variant(64, 2, "lean", "gh", "horner", True, 4)]  # lane per point, EG in LDS
-        for split, waves in ((4, 2), (4, 3), (6, 3), (8, 3), (8, 4)):   # reaction-split tiles
-            if 8 * (2 * ns * 64 + split * 64) <= LDS_BYTES:
-                for group in (4, 8):
+        # reaction-split tiles: (wavefronts per tile, waves/SIMD the registers are sized for),
+        # LDS re-read distances
+        for split, waves, groups in ((4, 2, (4, 8)), (4, 3, (2, 4)), (6, 3, (4, 8)), (8, 2, (4, 8)),
+                                     (8, 4, (2, 4, 8)), (16, 4, (2, 4))):
+            if 8 * (2 * ns * 64 + split * 64) <= LDS_BYTES and split <= mech.n_reactions:
+                for group in groups:
                     out.append(Variant(64, waves, "lean", "gh", "horner", False, group, split))
                 # the same with the coefficients compiled in
-                out.append(Variant(64, waves, "lean", "gh", "horner", False, 8, split, 0, True))
+                out.append(Variant(64, waves, "lean", "gh", "horner", False, groups[-1], split, 0, True))
         out.append(Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, True))
+        # lane per point with reactions in live-range order, coefficients read or compiled in
+        out += [Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True),
+                Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, True, True)]
     if table_supported(mech):
         for npt in (16, 32, 64):
             for waves, wpe in ((4, 1), (8, 2), (16, 4)):

@@ -49,6 +49,8 @@ LOG2E = 1.0 / math.log(2.0)
 LOG2_10 = math.log2(10.0)
 LOG10_2 = math.log10(2.0)
 
+LDS_PER_CU = 160 * 1024
+
 TEMPLATE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "templates")
 
 
@@ -79,12 +81,15 @@ class Variant:
                              # wavefronts, lanes = (point, reaction slot); for large mechanisms
     baked: bool = False      # compile the coefficient values into the code (no scalar loads that
                              # share the LGKM counter with LDS reads); the blob must then match
+    live: bool = False       # lane kernel: reactions ordered to keep few species live, each
+                             # species' state computed at its first use and its wdot stored after
+                             # its last (register pressure of the lane-per-point kernel)
 
     @property
     def tag(self) -> str:
         if self.table:
             return f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
-        c = "k" if self.baked else ""
+        c = ("k" if self.baked else "") + ("o" if self.live else "")
         if self.split > 1:
             return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
                     f"{self.poly[0]}g{self.group}{c}")
@@ -320,12 +325,14 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
             if len(r.troe) == 4:
                 fc += f" + cg_exp2(-cf[{o + 4}] * invT)"
             L.append(f"    const double fcent = {fc};")
-            L.append(f"    const double lf = cg_log2(fcent) * {dlit(LOG10_2)};  // log10 Fcent")
+            L.append("    const double l2f = cg_log2(fcent);")
+            L.append(f"    const double lf = l2f * {dlit(LOG10_2)};            // log10 Fcent")
             L.append(f"    const double lp = cg_log2(pr) * {dlit(LOG10_2)};     // log10 Pr")
             L.append("    const double cc = -0.4 - 0.67 * lf;")
             L.append("    const double nn = 0.75 - 1.27 * lf;")
-            L.append("    const double f1 = cg_div(lp + cc, nn - 0.14 * (lp + cc));")
-            L.append(f"    const double F = cg_exp2(cg_div({dlit(LOG2_10)} * lf, 1.0 + f1 * f1));")
+            # log10 F = log10 Fcent / (1 + (x/d)^2) = log10 Fcent d^2 / (d^2 + x^2): one division
+            L.append("    const double x = lp + cc, d = nn - 0.14 * x;")
+            L.append("    const double F = cg_exp2(cg_div(l2f * (d * d), __builtin_fma(x, x, d * d)));")
             L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr) * F;")
         elif r.rtype == "sri":
             L.append(f"    const double lp = cg_log2(pr) * {dlit(LOG10_2)};")
@@ -385,32 +392,63 @@ def reaction_cost(r: Reaction) -> float:
     return c
 
 
-def partition_reactions(mech: Mechanism, nparts: int) -> List[List[int]]:
-    """Split reactions over wavefronts: balance the cost, prefer the part whose reactions
-    already touch the same species (fewer partial-wdot registers per wavefront)."""
+def partition_reactions(mech: Mechanism, nparts: int, tol: float = 1.02, iters: int = 40000,
+                        seed: int = 0) -> List[List[int]]:
+    """Split reactions over wavefronts.  Every wavefront keeps one partial-wdot register pair
+    per species its reactions touch, so the split minimises the largest touched set under a
+    cost balance (each part at most ``tol`` x the mean cost): a greedy start (costliest
+    reaction first, into the part it adds the fewest new species to), then seeded random
+    single-reaction moves, kept when the touched-set sizes sorted in decreasing order do not
+    get worse (lexicographically)."""
+    import random
     if nparts <= 1:
         return [list(range(mech.n_reactions))]
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    S = [frozenset({idx[s] for s in r.reactants} | {idx[s] for s in r.products}) for r in mech.reactions]
     cost = [reaction_cost(r) for r in mech.reactions]
-    cap = sum(cost) / nparts * 1.03
+    cap = sum(cost) / nparts * tol
     load = [0.0] * nparts
-    touched = [set() for _ in range(nparts)]
+    where = [0] * mech.n_reactions
+    cnt: List[Dict[int, int]] = [dict() for _ in range(nparts)]
+
+    def put(j, v, sign):
+        load[v] += sign * cost[j]
+        for k in S[j]:
+            c = cnt[v].get(k, 0) + sign
+            if c:
+                cnt[v][k] = c
+            else:
+                del cnt[v][k]
+
+    for j in sorted(range(mech.n_reactions), key=lambda j: -cost[j]):
+        fits = [v for v in range(nparts) if load[v] + cost[j] <= cap]
+        v = (min(fits, key=lambda u: (len(S[j] - cnt[u].keys()), load[u])) if fits
+             else min(range(nparts), key=lambda u: load[u]))
+        where[j] = v
+        put(j, v, 1)
+
+    def score():
+        return sorted((len(c) for c in cnt), reverse=True)
+
+    rng = random.Random(seed)
+    best = score()
+    for _ in range(iters):
+        j = rng.randrange(mech.n_reactions)
+        a, b = where[j], rng.randrange(nparts)
+        if a == b or load[b] + cost[j] > cap:
+            continue
+        put(j, a, -1)
+        put(j, b, 1)
+        sc = score()
+        if sc <= best:
+            best, where[j] = sc, b
+        else:
+            put(j, b, -1)
+            put(j, a, 1)
     parts: List[List[int]] = [[] for _ in range(nparts)]
-    order = sorted(range(mech.n_reactions), key=lambda j: -cost[j])
-    for j in order:
-        r = mech.reactions[j]
-        sp = set(r.reactants) | set(r.products)
-        best = None
-        for v in range(nparts):
-            if load[v] + cost[j] > cap:
-                continue
-            key = (len(sp - touched[v]), load[v])
-            if best is None or key < best[0]:
-                best = (key, v)
-        v = best[1] if best else min(range(nparts), key=lambda u: load[u])
-        parts[v].append(j)
-        load[v] += cost[j]
-        touched[v] |= sp
-    return [sorted(p) for p in parts]
+    for j in range(mech.n_reactions):
+        parts[where[j]].append(j)
+    return parts
 
 
 def coef_hash(coef: np.ndarray) -> int:
@@ -435,6 +473,8 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
         return _emit_table(mech, variant)
     if variant.split > 1:
         return _emit_split(mech, variant)
+    if variant.live:
+        return _emit_live(mech, variant)
     import jinja2
     coef, off_smh, roffs = build_coef(mech)
     idx = {s.name: i for i, s in enumerate(mech.species)}
@@ -473,6 +513,98 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
     return src, coef, meta
 
 
+def live_order(mech: Mechanism) -> Tuple[List[int], int]:
+    """Reaction order for the lane kernel that keeps the set of live species small: greedily take
+    the reaction that brings in the fewest species not yet live minus the species it retires
+    (its last use).  A species (its EG, h and wdot registers) is live from its first to its last
+    reaction.  Returns (order, the largest live set)."""
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    S = [frozenset(idx[x] for x in list(r.reactants) + list(r.products) + list(r.third_body))
+         for r in mech.reactions]
+    remaining = [0] * mech.n_species
+    for sp in S:
+        for k in sp:
+            remaining[k] += 1
+    live: set = set()
+    left = set(range(mech.n_reactions))
+    order: List[int] = []
+    peak = 0
+    while left:
+        def score(j):
+            return (len(S[j] - live) - sum(1 for k in S[j] if remaining[k] == 1), j)
+        j = min(left, key=score)
+        left.remove(j)
+        order.append(j)
+        live |= S[j]
+        peak = max(peak, len(live))
+        for k in S[j]:
+            remaining[k] -= 1
+            if remaining[k] == 0:
+                live.discard(k)
+    return order, peak
+
+
+def _emit_live(mech: Mechanism, variant: Variant):
+    import jinja2
+    coef, off_smh, roffs = build_coef(mech)
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    ns = mech.n_species
+    order, peak = live_order(mech)
+    S = [set(idx[x] for x in list(r.reactants) + list(r.products) + list(r.third_body)) for r in mech.reactions]
+    last = {}
+    for pos, j in enumerate(order):
+        for k in S[j]:
+            last[k] = pos
+    tmids, group_of = [], []
+    for sp in mech.species:
+        if sp.thermo.t_mid not in tmids:
+            tmids.append(sp.thermo.t_mid)
+        group_of.append(tmids.index(sp.thermo.t_mid))
+    lines: List[str] = []
+    for k in range(ns):
+        if k not in last:
+            lines.append(f"  W[{k} * ld + i] = 0.0;  // species {mech.species[k].name} takes part in no reaction")
+    started: set = set()
+    for pos, j in enumerate(order):
+        for k in sorted(S[j] - started):
+            started.add(k)
+            lines += [f"  // species {k} ({mech.species[k].name}): state at its first use",
+                      f"  double g{k}, h{k};",
+                      "  {",
+                      f"    const double ck = Y[{k} * ld + i] * cf[{k}] * scale;",
+                      f"    const auto* a = cf + {off_smh + 14 * k};",
+                      "    double hl = a[6] * invT, hh = a[13] * invT;",
+                      "    hl = __builtin_fma(a[1], lg2T, hl); hh = __builtin_fma(a[8], lg2T, hh);",
+                      "    hl = __builtin_fma(a[2], t, hl);   hh = __builtin_fma(a[9], t, hh);",
+                      "    hl = __builtin_fma(a[3], t2, hl);  hh = __builtin_fma(a[10], t2, hh);",
+                      "    hl = __builtin_fma(a[4], t3, hl);  hh = __builtin_fma(a[11], t3, hh);",
+                      "    hl = __builtin_fma(a[5], t4, hl);  hh = __builtin_fma(a[12], t4, hh);",
+                      "    hl = hl + a[0];                    hh = hh + a[7];",
+                      f"    g{k} = cg_exp2_fin(lo{group_of[k]} ? hl : hh);",
+                      f"    h{k} = cg_div(ck, g{k});",
+                      "  }",
+                      f"  double w{k} = 0.0;"]
+        lines.extend(_reaction_code(j, mech.reactions[j], roffs[j], idx, "gh"))
+        for k in sorted(S[j]):
+            if last[k] == pos:
+                lines.append(f"  W[{k} * ld + i] = w{k};")
+    env = jinja2.Environment(loader=jinja2.FileSystemLoader(TEMPLATE_DIR), keep_trailing_newline=True)
+    tpl = env.get_template("getrates_live_hip.j2")
+    h = structure_hash(mech)
+    src = tpl.render(
+        name=mech.name, ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
+        waves_per_eu=variant.waves_per_eu, hash_hex=f"0x{h:016x}", peak=peak,
+        csum_expr=tree_sum([f"Y[{k} * ld + i] * cf[{k}]" for k in range(ns)]),
+        R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)),
+        patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, poly=variant.poly,
+        tmid_groups=[{"tmid": dlit(t)} for t in tmids], program_code="\n".join(lines),
+        threads=variant.threads, ppb=variant.ppb, **_baked_args(coef, variant),
+    )
+    meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
+                waves_per_eu=variant.waves_per_eu, structure_hash=h, peak_live=peak)
+    return src, coef, meta
+
+
 def _emit_split(mech: Mechanism, variant: Variant):
     import jinja2
     coef, off_smh, roffs = build_coef(mech)
@@ -480,6 +612,11 @@ def _emit_split(mech: Mechanism, variant: Variant):
     ns = mech.n_species
     W = variant.split
     parts = partition_reactions(mech, W)
+    # the per-wave partial sums get LDS rows of their own unless that LDS would keep the
+    # workgroups per CU below the occupancy the registers are sized for
+    lds_full = 8 * (2 * ns * 64 + W * 64)
+    compact_lds = (W <= ns and (LDS_PER_CU // lds_full) * W < 4 * variant.waves_per_eu
+                   and (LDS_PER_CU // (8 * 2 * ns * 64)) > LDS_PER_CU // lds_full)
     codes, touched = [], []
     for v in range(W):
         lines: List[str] = []
@@ -508,11 +645,12 @@ def _emit_split(mech: Mechanism, variant: Variant):
         off_smh=off_smh, R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0),
         r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of, species_of=species_of,
-        zero_of=species_of, touched=touched, reactions_code=codes, **_baked_args(coef, variant),
+        zero_of=species_of, touched=touched, reactions_code=codes, compact_lds=compact_lds,
+        **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h,
-                touched_per_wave=[len(t) for t in touched])
+                touched_per_wave=[len(t) for t in touched], compact_lds=compact_lds)
     return src, coef, meta
 
 

@@ -1,7 +1,8 @@
 // Handle-based C ABI over generated getrates code objects (see include/chemgen_amd.h).
 //
 // A mechanism handle owns: the hipModule loaded from the generated code object, the
-// two kernels it exports (cg_getrates_kernel, cg_getrates_tiles_kernel), the device
+// kernels it exports (cg_getrates_kernel; cg_getrates_tiles_kernel, a persistent walk of a
+// tile list; cg_getrates_tile_kernel, one workgroup per tile of that list), the device
 // copy of the gpu_coef blob, and the tile list of the last prepared ragged batch.
 #include "chemgen_amd.h"
 
@@ -54,6 +55,7 @@ struct cg_mech {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
   hipFunction_t fn_tiles = nullptr;
+  hipFunction_t fn_tile1 = nullptr;  // one workgroup per tile (absent in older code objects)
   double* coef_d = nullptr;
   Meta meta{};
   int device = 0;
@@ -96,6 +98,10 @@ extern "C" int cg_mech_load(const char* path, const double* coef, int64_t n_coef
   e = hipMemcpyDtoH(&h->meta, meta_d, sizeof(Meta));
   if (e == hipSuccess) e = hipModuleGetFunction(&h->fn, h->mod, "cg_getrates_kernel");
   if (e == hipSuccess) e = hipModuleGetFunction(&h->fn_tiles, h->mod, "cg_getrates_tiles_kernel");
+  if (e == hipSuccess && hipModuleGetFunction(&h->fn_tile1, h->mod, "cg_getrates_tile_kernel") != hipSuccess) {
+    h->fn_tile1 = nullptr;
+    (void)hipGetLastError();
+  }
   if (e != hipSuccess) {
     (void)hipModuleUnload(h->mod);
     delete h;
@@ -229,7 +235,9 @@ extern "C" int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const 
   int64_t nt = h->n_tiles;
   void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d,
                   (void*)&h->tile_start_d, (void*)&h->tile_len_d, (void*)&nt, (void*)&ld};
-  CG_HIP(hipModuleLaunchKernel(h->fn_tiles, (unsigned)g, 1, 1, (unsigned)h->meta.block, 1, 1, 0,
+  // a grid that covers every tile runs the loop-free kernel (same tiles, one each)
+  hipFunction_t fn = (g == nt && h->fn_tile1) ? h->fn_tile1 : h->fn_tiles;
+  CG_HIP(hipModuleLaunchKernel(fn, (unsigned)g, 1, 1, (unsigned)h->meta.block, 1, 1, 0,
                                (hipStream_t)stream, args, nullptr));
   return CG_OK;
 }

@@ -24,7 +24,11 @@ def default_variants(mech: Mechanism) -> List[Variant]:
     """Variants compiled by the build: the tuned (or default) shape first, then the
     autotuner's candidates for the mechanisms it sweeps."""
     from .autotune import best_variant, candidate_variants
+    from .autotune import STRAIGHT_LINE_MAX_SPECIES
     out = [best_variant(mech)]
     if mech.name in TUNED:
         out += [v for v in candidate_variants(mech) if v not in out]
+    live = Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True)
+    if mech.n_species <= STRAIGHT_LINE_MAX_SPECIES and live not in out:
+        out.append(live)   # the live-range-ordered lane kernel, for its parity tests
     return out

@@ -71,10 +71,12 @@ int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_t ng, int64
 /* Ragged batch in one launch.  Patch b covers points [offsets[b], offsets[b+1]) of the
  * packed arrays (offsets is a HOST array of n_patches+1 entries, offsets[0] = 0,
  * non-decreasing).  The tile list is built on the host and uploaded into a buffer the
- * handle owns (grown on demand), then one persistent kernel walks it; the launch is
+ * handle owns (grown on demand), then one kernel launch covers it; the launch is
  * capturable into a hipGraph once the buffer is large enough.  `grid` = number of
- * persistent workgroups (0 = 8 per CU, negative = one workgroup per tile).  Reference: no counterpart (the reference
- * launches one kernel per array, hip_v1_cpp.j2:339-372). */
+ * persistent workgroups walking the tile list (0 = 8 per CU); negative, or at least the
+ * number of tiles, = one workgroup per tile (a loop-free kernel over the same list).
+ * Reference: no counterpart (the reference launches one kernel per array,
+ * hip_v1_cpp.j2:339-372). */
 int cg_getrates_batched(cg_mech_t h, const double* P, const double* T, const double* Y,
                         double* wdot, int64_t ld, const int64_t* offsets, int32_t n_patches,
                         int32_t grid, void* stream);

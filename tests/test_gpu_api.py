@@ -57,9 +57,11 @@ def test_generated_module_point_and_species_layouts():
     assert parity_stats(m, w_dev.cpu().numpy(), ref, qf, qb)["max_gross"] <= TOL_GROSS
 
 
+@pytest.mark.parametrize("grid", [7, -1])
 @pytest.mark.parametrize("name", ["h2_burke2012", "synth_gri53"])
-def test_ragged_batch_matches_oracle(name):
-    """Ragged patches (sizes 1..300, some empty) in one persistent launch."""
+def test_ragged_batch_matches_oracle(name, grid):
+    """Ragged patches (sizes 1..300, some empty) in one launch: 7 persistent workgroups
+    walking the tile list, or one workgroup per tile (the loop-free kernel)."""
     import torch
     from chemgen_amd.autotune import best_variant
     from chemgen_amd.runtime import DeviceMechanism
@@ -71,15 +73,47 @@ def test_ragged_batch_matches_oracle(name):
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     ng = int(offs[-1])
     P, T, Y = make_states(m.n_species, ng, seed=6)
-    print(f"[batch] {name} ng={ng} patches={len(sizes)}", flush=True)
+    print(f"[batch] {name} ng={ng} patches={len(sizes)} grid={grid}", flush=True)
     dm = DeviceMechanism(m, best_variant(m), build=False)
     d = torch.device("cuda:0")
     Pd, Td, Yd = (torch.from_numpy(x).to(d) for x in (P, T, Y))
     W = torch.full_like(Yd, float("nan"))
-    dm.getrates_batched_(Pd, Td, Yd, W, offs, grid=7)
+    dm.getrates_batched_(Pd, Td, Yd, W, offs, grid=grid)
     torch.cuda.synchronize()
     ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
     assert parity_stats(m, W.cpu().numpy(), ref, qf, qb)["max_gross"] <= TOL_GROSS
+
+
+@pytest.mark.parametrize("name,tags", [
+    ("synth_gri53", ("b64w1lghh", "b64w1lghhko", "s4w2lghhg8", "s4w3lghhg2", "s8w4lghhg4")),
+    ("gri30", ("t16s8w2lh", "b64w2lghhL4")),
+])
+def test_ragged_batch_kernel_families(name, tags):
+    """Both batched kernels (persistent walk, one workgroup per tile) of each kernel family
+    -- lane, live-range lane with compiled-in coefficients, reaction-split (also with the
+    compact LDS layout), table, EG in LDS -- against the oracle on ragged patches."""
+    import torch
+    from chemgen_amd.autotune import candidate_variants
+    from chemgen_amd.runtime import DeviceMechanism
+    m = load_bundled(name)
+    sizes = np.array([130, 0, 64, 1, 250, 63, 65])
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    ng = int(offs[-1])
+    P, T, Y = make_states(m.n_species, ng, seed=9)
+    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    d = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(d) for x in (P, T, Y))
+    by_tag = {v.tag: v for v in candidate_variants(m)}
+    for tag in tags:
+        dm = DeviceMechanism(m, by_tag[tag], build=False)
+        for grid in (3, -1):
+            print(f"[batch-family] {name} {tag} grid={grid}", flush=True)
+            W = torch.full_like(Yd, float("nan"))
+            dm.getrates_batched_(Pd, Td, Yd, W, offs, grid=grid)
+            torch.cuda.synchronize()
+            st = parity_stats(m, W.cpu().numpy(), ref, qf, qb)
+            assert st["finite"] and st["max_gross"] <= TOL_GROSS, (tag, grid, st)
+        dm.close()
 
 
 def test_batched_graph_replay():

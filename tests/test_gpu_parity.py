@@ -98,3 +98,22 @@ def test_baked_coefficients_vs_oracle_and_guard(torch_cuda):
         dm.set_coef(coef)
     with pytest.raises(ChemgenError):
         DeviceMechanism(m, v, build=False, coef=coef)
+
+
+@pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_nhept88", "synth_plog53"])
+def test_live_order_kernels_vs_oracle(torch_cuda, name):
+    """Lane kernels with reactions in live-range order (species state made at first use, wdot
+    stored after last use), with the coefficients read and, where built, compiled in."""
+    from chemgen_amd.codegen import Variant
+    from chemgen_amd.kernels import default_variants
+    from chemgen_amd.runtime import DeviceMechanism
+    m = load_bundled(name)
+    P, T, Y = make_states(m.n_species, 1000, seed=8, T_range=(300.0, 2500.0))
+    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    lives = [v for v in default_variants(m) if v.live]
+    assert Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True) in lives
+    for v in lives:
+        print(f"[live] {name} {v.tag}", flush=True)
+        w = _run(torch_cuda, m, P, T, Y, DeviceMechanism(m, v, build=False))
+        st = parity_stats(m, w, ref, qf, qb)
+        assert st["finite"] and st["max_gross"] <= TOL_GROSS, (v.tag, st)

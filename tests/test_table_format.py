@@ -40,3 +40,32 @@ def test_table_decodes_to_mechanism(name, npt):
 def test_plog_mechanism_is_not_tabled():
     assert not table_supported(load_bundled("synth_plog53"))
     assert table_supported(load_bundled("synth_jetsurf348"))
+
+
+@pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_plog53"])
+def test_live_order_is_a_permutation_with_its_peak(name):
+    """The live-range order visits every reaction once, and the peak it reports is the largest
+    set of species between their first and last reaction, recounted here from first/last
+    positions; for GRI-Mech 3.0 it keeps fewer species live than the file order."""
+    from chemgen_amd.codegen import live_order
+    m = load_bundled(name)
+    order, peak = live_order(m)
+    assert sorted(order) == list(range(m.n_reactions))
+    idx = {s.name: i for i, s in enumerate(m.species)}
+    first, last = {}, {}
+    for pos, j in enumerate(order):
+        r = m.reactions[j]
+        for s in list(r.reactants) + list(r.products) + list(r.third_body):
+            first.setdefault(idx[s], pos)
+            last[idx[s]] = pos
+    recount = max(sum(1 for k in first if first[k] <= pos <= last[k]) for pos in range(len(order)))
+    assert recount == peak
+    if name == "gri30":
+        fo = {}
+        lo = {}
+        for pos, r in enumerate(m.reactions):
+            for s in list(r.reactants) + list(r.products) + list(r.third_body):
+                fo.setdefault(idx[s], pos)
+                lo[idx[s]] = pos
+        file_peak = max(sum(1 for k in fo if fo[k] <= pos <= lo[k]) for pos in range(m.n_reactions))
+        assert peak < file_peak
