@@ -14,6 +14,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+WARM_S = 1.0   # untimed warm-up time before each timed region, seconds
+
+
+def warm_up(launch, warmup, seconds=WARM_S):
+    """At least ``warmup`` untimed launches, more until ``seconds`` have passed: the timed
+    launches then run at the clocks a sustained load settles at, not at idle clocks.
+    Returns the number of launches made."""
+    import torch
+    n = 0
+    t0 = time.perf_counter()
+    while n < warmup or time.perf_counter() - t0 < seconds:
+        launch()
+        n += 1
+        if n % 8 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    return n
+
+
 def cpu_baseline(mech_name, mech, target_s=10.0, sample_ng=1 << 14):
     """C/OpenMP oracle on the host cores over a bounded sample of the same states."""
     import numpy as np
@@ -67,9 +86,7 @@ def bench_batched(steps, warmup):
             dm.getrates_batched_prepared_(Pd, Td, Yd, Wd, grid=grid, stream=s)
     torch.cuda.synchronize()
     with torch.cuda.stream(s):
-        for _ in range(warmup):
-            g.replay()
-        torch.cuda.synchronize()
+        warm_up(g.replay, warmup)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(s)
         for _ in range(steps):
@@ -123,9 +140,7 @@ def bench_large(steps, warmup, ng=1 << 19):
     Wd = torch.empty_like(Yd)
     dm = DeviceMechanism(mech, variant, build=False)
     s = torch.cuda.current_stream()
-    for _ in range(warmup):
-        dm.getrates_(Pd, Td, Yd, Wd, s)
-    torch.cuda.synchronize()
+    warm_up(lambda: dm.getrates_(Pd, Td, Yd, Wd, s), warmup)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
     for _ in range(steps):
@@ -175,9 +190,7 @@ def bench_surrogate(steps, warmup, ng=1 << 20):
     s = torch.cuda.current_stream()
 
     def timed(fn):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
+        warm_up(fn, warmup)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(s)
         for _ in range(steps):
@@ -264,9 +277,7 @@ def main():
     Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
     Wd = torch.empty_like(Yd)
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        dm.getrates_(Pd, Td, Yd, Wd, stream)
-    torch.cuda.synchronize()
+    n_warm = warm_up(lambda: dm.getrates_(Pd, Td, Yd, Wd, stream), args.warmup)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t0 = time.perf_counter()
     evs[0].record(stream)
@@ -304,6 +315,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_launches": n_warm,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -311,7 +323,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.mech} ({mech.n_species} species, {mech.n_reactions} reactions), "
-                               f"ng={ng}, T U(300,2500) K, P 1-40 atm log-uniform, Y Dirichlet(0.5), seed 0",
+                               f"ng={ng}, T U(300,2500) K, P 1-40 atm log-uniform, Y Dirichlet(0.5), seed 0; "
+                               f"at least {WARM_S:g} s of untimed launches before the timed ones",
                    "mechanism": args.mech, "ng": ng, "kernel_variant": dm.variant.tag},
         "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": PEAK_FP64_VECTOR / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VECTOR, "traffic": traffic,
