@@ -160,6 +160,75 @@ def bench_large(steps, warmup, ng=1 << 19):
             "parity": {"sample_points": nchk, **par}}
 
 
+# the reference's only published getrates numbers: H2 Burke mini-app on one gfx90a GCD
+# (performance_table_mini_apps_H2_burke.csv:213, ngg = 2,097,152 points, best HIP variant;
+# csv:222, one CPU core, getrates_i, ng = 262,144)
+H2_PUBLISHED_GPU = 7.69e8
+H2_PUBLISHED_CPU_CORE = 5.06e6
+
+
+def bench_h2_miniapp(steps, warmup, ngg=1 << 21, ng_cpu=1 << 18):
+    """The published H2 Burke mini-app measurement redone here: getrates over ngg = 2^21
+    points on the GPU (tuned variant, HIP events, inputs resident) and one host core running
+    the C restatement of getrates_i over 2^18 points; T U(300, 2500) K, P 1-40 atm, Y
+    Dirichlet(0.5), seed 0.  The published figures are from other hardware (gfx90a GCD, an
+    unstated CPU) and are shown for reference only."""
+    import numpy as np
+    import torch
+    from chemgen_amd.autotune import best_variant
+    from chemgen_amd.kernels import load_bundled
+    from chemgen_amd.roofline import PEAK_FP64_VECTOR, PEAK_HBM, work_per_point
+    from chemgen_amd.runtime import DeviceMechanism
+    from chemgen_amd.synth import make_states
+    from oracle.build import COracle
+    from oracle.getrates_np import getrates as np_oracle
+    from oracle.metrics import parity_stats
+    name = "h2_burke"
+    mech = load_bundled(name)
+    variant = best_variant(mech)
+    P, T, Y = make_states(mech.n_species, ngg, seed=0)
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
+    Wd = torch.empty_like(Yd)
+    dm = DeviceMechanism(mech, variant, build=False)
+    s = torch.cuda.current_stream()
+    warm_up(lambda: dm.getrates_(Pd, Td, Yd, Wd, s), warmup)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        dm.getrates_(Pd, Td, Yd, Wd, s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / steps / 1e3
+    nchk = 4096
+    ref, qf, qb = np_oracle(mech, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
+    par = parity_stats(mech, Wd[:, :nchk].cpu().numpy(), ref, qf, qb)
+    # one host core, C restatement of getrates_i
+    co = COracle(name)
+    threads0 = co.threads
+    co.set_threads(1)
+    Pc, Tc, Yc = P[:ng_cpu].copy(), T[:ng_cpu].copy(), np.ascontiguousarray(Y[:, :ng_cpu])
+    co.getrates(Pc, Tc, Yc)
+    reps, t0 = 0, time.perf_counter()
+    while reps < 3 or time.perf_counter() - t0 < 2.0:
+        co.getrates(Pc, Tc, Yc)
+        reps += 1
+    t_cpu = (time.perf_counter() - t0) / reps
+    co.set_threads(threads0)
+    w = work_per_point(mech)
+    achieved_bw = w["bytes"] * ngg / t
+    return {"workload": f"{name} ({mech.n_species} species, {mech.n_reactions} reactions), ngg={ngg}, "
+                        f"T U(300,2500) K, P 1-40 atm log-uniform, Y Dirichlet(0.5), seed 0 -- the published "
+                        f"mini-app size",
+            "points_per_s": ngg / t, "kernel_avg_ms": t * 1e3, "kernel_variant": variant.tag,
+            "published_gfx90a_points_per_s": H2_PUBLISHED_GPU,
+            "hbm_frac": achieved_bw / PEAK_HBM, "fp64_frac": w["flops"] * ngg / t / PEAK_FP64_VECTOR,
+            "cpu_one_core_points_per_s": ng_cpu / t_cpu,
+            "cpu_one_core_sample": f"{reps} x {ng_cpu} points, C restatement of getrates_i, 1 OpenMP thread",
+            "published_cpu_one_core_points_per_s": H2_PUBLISHED_CPU_CORE,
+            "parity": {"sample_points": nchk, **par}}
+
+
 def bench_surrogate(steps, warmup, ng=1 << 20):
     """The learned surrogate for the GRI-sized mechanism (4 x 256 SiLU MLP, BF16 on the matrix
     cores, weights in chemgen_amd/data) on 2^20 held-out states (T U(800, 2800) K, seed 7):
@@ -244,6 +313,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
     ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate workload")
+    ap.add_argument("--no-h2", action="store_true", help="skip the H2 Burke mini-app workload")
     ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
                     help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
@@ -342,6 +412,9 @@ def main():
     if not args.no_surrogate:
         torch.cuda.empty_cache()
         out["surrogate"] = bench_surrogate(args.steps, args.warmup)
+    if not args.no_h2:
+        torch.cuda.empty_cache()
+        out["h2_miniapp"] = bench_h2_miniapp(args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
     print(json.dumps(_finite(out)))

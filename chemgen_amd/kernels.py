@@ -17,7 +17,7 @@ def load_bundled(name: str) -> Mechanism:
     return parse_chemkin(os.path.join(DATA_DIR, name + ".inp"), name=name)
 
 
-TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348")
+TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348", "h2_burke")
 
 
 def default_variants(mech: Mechanism) -> List[Variant]:

@@ -57,6 +57,11 @@ class COracle:
         self.ns = self.lib.oracle_n_species()
         self.threads = self.lib.oracle_threads()
 
+    def set_threads(self, n: int):
+        """OpenMP threads of the following calls (process-wide)."""
+        self.lib.oracle_set_threads(int(n))
+        self.threads = self.lib.oracle_threads()
+
     def getrates(self, P, T, Y):
         import numpy as np
         P = np.ascontiguousarray(P, dtype=np.float64)

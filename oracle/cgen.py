@@ -160,6 +160,7 @@ def emit_c(mech, vl: int = 32) -> str:
     a("")
     a("int oracle_n_species(void) { return NS; }")
     a("int oracle_threads(void) { return omp_get_max_threads(); }")
+    a("void oracle_set_threads(int n) { omp_set_num_threads(n); }")
     a("void oracle_getrates(int64_t ng, int64_t ld, const double* P, const double* T, const double* Y,")
     a("                     double* W) {")
     a("  #pragma omp parallel for schedule(static)")

@@ -82,10 +82,11 @@ def test_table_kernels_vs_oracle(torch_cuda, name):
 def test_baked_coefficients_vs_oracle_and_guard(torch_cuda):
     """A kernel with the coefficients compiled in: parity with the oracle, and a blob that
     differs from the compiled one is refused at load and at cg_mech_set_coef."""
-    from chemgen_amd.codegen import Variant, build_coef
+    from chemgen_amd.autotune import candidate_variants
+    from chemgen_amd.codegen import build_coef
     from chemgen_amd.runtime import ChemgenError, DeviceMechanism
     m = load_bundled("synth_gri53")
-    v = Variant(64, 2, "lean", "gh", "horner", False, 8, 4, 0, True)
+    v = next(v for v in candidate_variants(m) if v.baked and v.split == 4)
     P, T, Y = make_states(m.n_species, 2048, seed=6)
     dm = DeviceMechanism(m, v, build=False)
     w = _run(torch_cuda, m, P, T, Y, dm)
