@@ -217,12 +217,20 @@ def bench_h2_miniapp(steps, warmup, ngg=1 << 21, ng_cpu=1 << 18):
     co.set_threads(threads0)
     w = work_per_point(mech)
     achieved_bw = w["bytes"] * ngg / t
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh).get(f"{name}/{ngg}/{variant.tag}")
+        if tr:
+            traffic = tr["read_bytes"] + tr["write_bytes"]
     return {"workload": f"{name} ({mech.n_species} species, {mech.n_reactions} reactions), ngg={ngg}, "
                         f"T U(300,2500) K, P 1-40 atm log-uniform, Y Dirichlet(0.5), seed 0 -- the published "
                         f"mini-app size",
             "points_per_s": ngg / t, "kernel_avg_ms": t * 1e3, "kernel_variant": variant.tag,
             "published_gfx90a_points_per_s": H2_PUBLISHED_GPU,
             "hbm_frac": achieved_bw / PEAK_HBM, "fp64_frac": w["flops"] * ngg / t / PEAK_FP64_VECTOR,
+            "traffic": traffic,
             "cpu_one_core_points_per_s": ng_cpu / t_cpu,
             "cpu_one_core_sample": f"{reps} x {ng_cpu} points, C restatement of getrates_i, 1 OpenMP thread",
             "published_cpu_one_core_points_per_s": H2_PUBLISHED_CPU_CORE,
