@@ -657,21 +657,75 @@ def _emit_split(mech: Mechanism, variant: Variant):
 _TABLE_KIND = {"standard": 0, "third_body": 1, "lindemann": 2, "troe": 3, "sri": 5}
 
 
-def table_order(mech: Mechanism) -> List[int]:
-    """Processing order of the table kernel: reactions grouped by class (kind, reversibility,
-    species counts, efficiency count), so that consecutive reaction slots -- the lane groups
-    of one wavefront -- take the same branches."""
-    def key(j):
-        r = mech.reactions[j]
-        return (_TABLE_KIND[r.rtype], r.reversible, len(r.reactants), len(r.products), len(r.third_body), j)
-    return sorted(range(mech.n_reactions), key=key)
+def _table_slots(r: Reaction, idx: Dict[str, int]) -> Tuple[List[int], List[int]]:
+    out = []
+    for side in (r.reactants, r.products):
+        sl: List[int] = []
+        for sp, nu in side.items():
+            if not float(nu).is_integer():
+                raise ValueError(f"table kernel: stoichiometric coefficient {nu}")
+            sl += [idx[sp]] * int(nu)
+        if not 1 <= len(sl) <= 3:
+            raise ValueError("table kernel: a reaction side needs 1 to 3 species slots")
+        out.append(sl)
+    return out[0], out[1]
 
 
-def table_records(mech: Mechanism, npt: int = 16):
-    """(reaction table rows [2 nr][4], efficiency species, order) for the table kernel with
-    ``npt``-point tiles.  Each side of a reaction is written as up to three species *slots*
-    (a species with coefficient nu fills nu slots), absent slots point at an extra LDS row
-    that holds EG = h = 1; slots are stored as LDS row offsets (species index x npt).
+def _table_kind(r: Reaction) -> int:
+    kind = _TABLE_KIND[r.rtype]
+    if r.rtype == "troe" and len(r.troe) == 4:
+        kind = 4
+    if r.rtype == "sri" and len(r.sri) == 5 and r.sri[4] != 0.0:
+        kind = 6
+    return kind
+
+
+def table_class(mech: Mechanism, j: int) -> Tuple[int, int, int, int, int]:
+    """The shape the table kernel compiles a loop for: (family, reversible, reactant slots,
+    product slots, dnu if reversible else 0); family 0 elementary, 1 third-body,
+    2 Lindemann, 3 Troe (with or without T2), 5 SRI (with or without e)."""
+    r = mech.reactions[j]
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    rs, ps = _table_slots(r, idx)
+    fam = _TABLE_KIND[r.rtype]
+    return (fam, int(r.reversible), len(rs), len(ps), (len(ps) - len(rs)) if r.reversible else 0)
+
+
+def table_order(mech: Mechanism, min_class: int = 0):
+    """Processing order of the table kernel and its class segments.  Reactions of a class
+    (``table_class``) with at least ``min_class`` members are contiguous and get a loop of
+    their own, compiled for that shape; the rest follow in one generic segment, sorted by
+    (kind, reversibility, slot counts, efficiency count) so that the lane groups of one
+    wavefront mostly take the same branches.  Returns (order, [(start, end, class)...],
+    start of the generic segment)."""
+    by: Dict[Tuple, List[int]] = {}
+    for j in range(mech.n_reactions):
+        by.setdefault(table_class(mech, j), []).append(j)
+    order: List[int] = []
+    segs = []
+    for key in sorted(by):
+        if len(by[key]) >= max(1, min_class):
+            segs.append((len(order), len(order) + len(by[key]), key))
+            order.extend(by[key])
+    rest = sorted((j for key in by if len(by[key]) < max(1, min_class) for j in by[key]),
+                  key=lambda j: (_table_kind(mech.reactions[j]), mech.reactions[j].reversible,
+                                 table_class(mech, j)[2:4], len(mech.reactions[j].third_body), j))
+    gen_start = len(order)
+    order.extend(rest)
+    return order, segs, gen_start
+
+
+def table_records(mech: Mechanism, npt: int = 16, min_class: int = 0):
+    """(reaction table rows [2 nr][4], efficiency species, order, class segments, start of the
+    generic segment) for the table kernel with ``npt``-point tiles.  Each side of a reaction is
+    written as up to three species *slots* (a species with coefficient nu fills nu slots),
+    absent slots point at an extra LDS row that holds EG = h = 1; slots are stored as LDS row
+    offsets (species index x npt).
+      row 0: x = blob offset of the reaction's record, y = r0 | r1 << 16, z = r2 | p0 << 16,
+             w = p1 | p2 << 16
+      row 1: x = kind | reversible << 3 | reactant slots << 4 | product slots << 6
+                 | efficiencies << 8 | (dnu + 8) << 24, y = offset of the collision-efficiency
+             species in the efficiency list, z = blob offset of their (eff - 1) values, w = 0
     Raises ValueError for what the format does not hold (more than 3 slots per side,
     non-integer coefficients, more than 15 efficiencies, PLOG)."""
     coef, off_smh, roffs = build_coef(mech)
@@ -682,50 +736,46 @@ def table_records(mech: Mechanism, npt: int = 16):
     for r in mech.reactions:
         if r.rtype not in _TABLE_KIND:
             raise ValueError(f"table kernel: reaction type {r.rtype} not supported")
-    rows: List[Tuple[int, int, int, int]] = []
-    effsp: List[int] = []
-    order = table_order(mech)
-    for j in order:
-        r = mech.reactions[j]
-        kind = _TABLE_KIND[r.rtype]
-        if r.rtype == "troe" and len(r.troe) == 4:
-            kind = 4
-        if r.rtype == "sri" and len(r.sri) == 5 and r.sri[4] != 0.0:
-            kind = 6
         if len(r.third_body) > 15:
             raise ValueError("table kernel: more than 15 collision efficiencies")
-        slots = []
-        for side in (r.reactants, r.products):
-            sl = []
-            for sp, nu in side.items():
-                if not float(nu).is_integer():
-                    raise ValueError(f"table kernel: stoichiometric coefficient {nu}")
-                sl += [idx[sp]] * int(nu)
-            if not 1 <= len(sl) <= 3:
-                raise ValueError("table kernel: a reaction side needs 1 to 3 species slots")
-            slots.append(sl)
-        nrs, nps = len(slots[0]), len(slots[1])
+    rows: List[Tuple[int, int, int, int]] = []
+    effsp: List[int] = []
+    order, segs, gen_start = table_order(mech, min_class)
+    for j in order:
+        r = mech.reactions[j]
+        kind = _table_kind(r)
+        rsl, psl = _table_slots(r, idx)
+        nrs, nps = len(rsl), len(psl)
         dnu = nps - nrs
         x = kind | (int(r.reversible) << 3) | (nrs << 4) | (nps << 6) | (len(r.third_body) << 8) \
             | ((dnu + 8) << 24)
-        rr = [k * npt for k in slots[0]] + [pad * npt] * (3 - nrs)
-        pp = [k * npt for k in slots[1]] + [pad * npt] * (3 - nps)
+        rr = [k * npt for k in rsl] + [pad * npt] * (3 - nrs)
+        pp = [k * npt for k in psl] + [pad * npt] * (3 - nps)
         nx = {"standard": 0, "third_body": 0, "lindemann": 3, "troe": 3 + (5 if len(r.troe or ()) == 4 else 4),
               "sri": 3 + 5}[r.rtype]
-        rows.append((x, roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16)))
-        rows.append((pp[1] | (pp[2] << 16), len(effsp), roffs[j] + 3 + nx, 0))
+        rows.append((roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16), pp[1] | (pp[2] << 16)))
+        rows.append((x, len(effsp), roffs[j] + 3 + nx, 0))
         effsp.extend(idx[sp] for sp in r.third_body)
-    return rows, effsp, order
+    return rows, effsp, order, segs, gen_start
 
 
 def _emit_table(mech: Mechanism, variant: Variant):
     import jinja2
     coef, off_smh, roffs = build_coef(mech)
     npt, W = variant.table, variant.split
-    rows, effsp, order = table_records(mech, npt)
     if 64 % npt:
         raise ValueError("table kernel: points per tile must divide 64")
     slots = W * 64 // npt
+    # a class gets a compiled loop when it fills every reaction slot of a tile at least once
+    rows, effsp, order, segs, gen_start = table_records(mech, npt, min_class=slots)
+    classes = []
+    for (a, b, (fam, rev, nrs, nps, dnu)) in segs:
+        pf = None
+        if rev:
+            pf = {0: None, 1: "pinv", -1: "pfac"}.get(dnu, f"cg_exp2({dlit(-float(dnu))} * lg2pfac)")
+        classes.append(dict(start=a, end=b, fam=fam, rev=rev, nrs=nrs, nps=nps, dnu=dnu, pf=pf,
+                            name=f"family {fam}, {'reversible' if rev else 'irreversible'}, "
+                                 f"{nrs} -> {nps} slots, dnu {dnu}"))
     env = jinja2.Environment(loader=jinja2.FileSystemLoader(TEMPLATE_DIR), keep_trailing_newline=True)
     tpl = env.get_template("getrates_table_hip.j2")
     h = structure_hash(mech)
@@ -737,10 +787,12 @@ def _emit_table(mech: Mechanism, variant: Variant):
         lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
         log10_2=dlit(LOG10_2), log2_10=dlit(LOG2_10), exp_impl=variant.exp_impl, poly=variant.poly,
         rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
+        classes=classes, gen_start=gen_start,
         tmid=", ".join(dlit(s.thermo.t_mid) for s in mech.species),
     )
     meta = dict(ns=mech.n_species, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads,
-                ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, structure_hash=h, slots=slots)
+                ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, structure_hash=h, slots=slots,
+                compiled_classes=len(classes), generic_reactions=mech.n_reactions - gen_start)
     return src, coef, meta
 
 

@@ -5,15 +5,15 @@ from collections import Counter
 import pytest
 
 from chemgen_amd.autotune import table_supported
-from chemgen_amd.codegen import build_coef, table_records
+from chemgen_amd.codegen import build_coef, table_class, table_records
 from chemgen_amd.kernels import load_bundled
 
 
 @pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_jetsurf348"])
-@pytest.mark.parametrize("npt", [16, 64])
-def test_table_decodes_to_mechanism(name, npt):
+@pytest.mark.parametrize("npt,min_class", [(16, 64), (64, 0), (16, 1000)])
+def test_table_decodes_to_mechanism(name, npt, min_class):
     m = load_bundled(name)
-    rows, effsp, order = table_records(m, npt)
+    rows, effsp, order, segs, gen_start = table_records(m, npt, min_class)
     _, _, roffs = build_coef(m)
     pad = m.n_species * npt
     idx = {s.name: i for i, s in enumerate(m.species)}
@@ -21,7 +21,7 @@ def test_table_decodes_to_mechanism(name, npt):
     kinds = {"standard": {0}, "third_body": {1}, "lindemann": {2}, "troe": {3, 4}, "sri": {5, 6}}
     for n, j in enumerate(order):
         r = m.reactions[j]
-        (x, y, z, w), (x2, y2, z2, _) = rows[2 * n], rows[2 * n + 1]
+        (y, z, w, x2), (x, e0, e1, _) = rows[2 * n], rows[2 * n + 1]
         assert (x & 7) in kinds[r.rtype]
         assert ((x >> 3) & 1) == int(r.reversible)
         nrs, nps, ne, dnu = (x >> 4) & 3, (x >> 6) & 3, (x >> 8) & 15, ((x >> 24) & 15) - 8
@@ -34,7 +34,16 @@ def test_table_decodes_to_mechanism(name, npt):
             assert Counter(s // npt for s in slots[:count]) == Counter({idx[k]: int(v) for k, v in side.items()})
         assert dnu == int(r.coeff_sum)
         assert ne == len(r.third_body)
-        assert effsp[y2:y2 + ne] == [idx[k] for k in r.third_body]
+        assert effsp[e0:e0 + ne] == [idx[k] for k in r.third_body]
+    # class segments: contiguous, one shape each, at least min_class long, then the generic rest
+    pos = 0
+    for a, b, key in segs:
+        assert a == pos and b - a >= max(1, min_class)
+        assert all(table_class(m, order[n]) == key for n in range(a, b))
+        pos = b
+    assert gen_start == pos
+    seg_keys = {key for _, _, key in segs}
+    assert all(table_class(m, order[n]) not in seg_keys for n in range(gen_start, m.n_reactions))
 
 
 def test_plog_mechanism_is_not_tabled():
