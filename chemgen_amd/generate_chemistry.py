@@ -11,7 +11,9 @@
   <stem>.coef.npy    the gpu_coef blob
   <stem>.json, .inp  mechanism metadata and the mechanism as parsed
 ``--mode gpu_coef`` writes DIR/gpu_coef_cdna4.npy (+ .json): the coefficient blob the
-kernels read (the layout is documented in chemgen_amd/codegen.py).
+kernels read (the layout is documented in chemgen_amd/codegen.py); with ``--compat
+reference`` also DIR/coef_m_vN.f90 (N = --parallel-level, at most 3), the reference's
+Fortran coefficient module that its ``chemgen_m`` uses (chemgen_amd/legacy_coef.py).
 
 Launch shape, the analogue of the reference's parallel levels (--parallel-level,
 --veclen, --nreact-per-block):
@@ -26,8 +28,8 @@ reference's C entry points for this mechanism (copyConstantsToDevice taking the 
 coefficient arrays, getrates_hip_wrapper_v1..v4, allocate_intermediate_DeviceMemory) over the
 generated kernel, so that the reference's Fortran module links it unchanged.
 
-The reference's python/fortran/OpenMP targets, its Fortran coefficient module and the
-mini-app are not produced here (those targets exist in the reference).
+The reference's python/fortran/OpenMP targets, its constants_vN.h (read only by its own HIP
+kernels) and the mini-app are not produced here (those targets exist in the reference).
 """
 from __future__ import annotations
 
@@ -102,6 +104,14 @@ def generate_gpu_coefficients(args) -> None:
         json.dump({"mechanism": mech.name, "species": mech.species_names, "n_coef": len(coef),
                    "smh_offset": off_smh, "reaction_offsets": roffs,
                    "structure_hash": f"{structure_hash(mech):016x}"}, fh, indent=1)
+    if args.compat == "reference":
+        # the reference's Fortran coefficient module, for chemgen_m over the legacy library
+        from .legacy_coef import emit_coef_module
+        level = min(args.parallel_level, 3)
+        path = os.path.join(args.output, f"coef_m_v{level}.f90")
+        with open(path, "w") as fh:
+            fh.write(emit_coef_module(mech, level))
+        print(f"reference coefficient module written to {path}")
     print(f"gpu_coef written to {args.output}")
 
 

@@ -1,0 +1,58 @@
+"""The reference's Fortran coefficient module written by this build (chemgen_amd/legacy_coef.py)
+holds, under the reference's names and sizes, exactly the arrays that the oracle's restatement
+of write_constants.py gives (oracle/legacy_coef.py), for every bundled mechanism whose
+reaction types the reference's HIP path holds."""
+import re
+
+import numpy as np
+import pytest
+
+from chemgen_amd.kernels import load_bundled
+from chemgen_amd.legacy_coef import emit_coef_module
+from oracle.legacy_coef import legacy_arrays
+
+
+def parse_module(src):
+    """{name: (fortran type, values)} from ``type :: name(n)=(/& ... /)`` blocks."""
+    out = {}
+    for m in re.finditer(r"^\s*(real\(kind=8\)|integer) :: (\w+)\((\d+)\)=\(/&\n(.*?)^\s*/\)", src,
+                         re.S | re.M):
+        ftype, name, n, body = m.group(1), m.group(2), int(m.group(3)), m.group(4)
+        toks = [t.strip() for t in body.replace("&", "").replace("\n", " ").split(",") if t.strip()]
+        assert len(toks) == n, name
+        vals = [int(t) for t in toks] if ftype == "integer" else [float(t.replace("D", "E")) for t in toks]
+        out[name] = (ftype, vals)
+    return out
+
+
+@pytest.mark.parametrize("name", ["h2_burke", "h2_burke2012"])
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_coef_module_matches_reference_arrays(name, level):
+    m = load_bundled(name)
+    src = emit_coef_module(m, level)
+    assert src.startswith(f"module coef_m_v{level}\n") and f"end module coef_m_v{level}" in src
+    assert max(len(l) for l in src.splitlines()) <= 132       # free-form line limit
+    got = parse_module(src)
+    ref = legacy_arrays(m)
+    assert set(got) == set(ref)
+    for k, v in ref.items():
+        ftype, vals = got[k]
+        assert ftype == ("integer" if v.dtype.kind == "i" else "real(kind=8)"), k
+        assert np.array_equal(np.asarray(vals, dtype=v.dtype), v), k
+
+
+def test_coef_module_refuses_what_the_reference_cannot_hold():
+    with pytest.raises(ValueError):
+        emit_coef_module(load_bundled("gri30"))          # Lindemann reactions
+    with pytest.raises(ValueError):
+        emit_coef_module(load_bundled("h2_burke"), 4)    # level 4 passes rocBLAS matrices
+
+
+def test_generator_writes_the_module(tmp_path):
+    import os
+    from chemgen_amd.generate_chemistry import main
+    from chemgen_amd.kernels import DATA_DIR
+    main(["--mode", "gpu_coef", "--mech", os.path.join(DATA_DIR, "h2_burke.inp"), "--output", str(tmp_path),
+          "--compat", "reference", "--parallel-level", "2"])
+    src = (tmp_path / "coef_m_v2.f90").read_text()
+    assert parse_module(src)["A_h"][1] == parse_module(emit_coef_module(load_bundled("h2_burke"), 2))["A_h"][1]
