@@ -118,3 +118,52 @@ def test_live_order_kernels_vs_oracle(torch_cuda, name):
         w = _run(torch_cuda, m, P, T, Y, DeviceMechanism(m, v, build=False))
         st = parity_stats(m, w, ref, qf, qb)
         assert st["finite"] and st["max_gross"] <= TOL_GROSS, (v.tag, st)
+
+
+def _edge_states(m):
+    """Crafted states: T at 300 K, at and next to each NASA-7 range switch, at 3000 and
+    3500 K (beyond the fitted range); P from 0.01 to 100 atm; compositions that are a single
+    species, sparse (most mass fractions exactly 0), and uniform."""
+    rng = np.random.default_rng(11)
+    ns = m.n_species
+    tmids = sorted({s.thermo.t_mid for s in m.species})
+    temps = [300.0, 3000.0, 3500.0] + [x for t in tmids for x in (t, np.nextafter(t, 0.0), np.nextafter(t, 1e9))]
+    pres = [0.01 * 1013250.0, 1013250.0, 100.0 * 1013250.0]
+    comps = []
+    for k in (0, ns // 2, ns - 1):
+        y = np.zeros(ns); y[k] = 1.0; comps.append(y)
+    for _ in range(3):
+        y = rng.dirichlet(0.3 * np.ones(ns)); y[y < np.quantile(y, 0.8)] = 0.0; comps.append(y / y.sum())
+    comps.append(np.full(ns, 1.0 / ns))
+    P, T, Y = [], [], []
+    for t in temps:
+        for p in pres:
+            for y in comps:
+                P.append(p); T.append(t); Y.append(y)
+    return np.array(P), np.array(T), np.ascontiguousarray(np.array(Y).T)
+
+
+@pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_nhept88", "synth_jetsurf348",
+                                  "synth_plog53"])
+def test_edge_states_vs_oracle(torch_cuda, name):
+    """The tuned kernel of each mechanism on the crafted states of _edge_states."""
+    m = load_bundled(name)
+    P, T, Y = _edge_states(m)
+    print(f"[edge] {name} ng={P.shape[0]}", flush=True)
+    w = _run(torch_cuda, m, P, T, Y)
+    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    st = parity_stats(m, w, ref, qf, qb)
+    assert st["finite"] and st["max_gross"] <= TOL_GROSS, st
+
+
+def test_zero_points(torch_cuda):
+    """ng = 0 launches nothing and is not an error."""
+    from chemgen_amd.runtime import DeviceMechanism
+    m = load_bundled("synth_gri53")
+    dm = DeviceMechanism(m, build=False)
+    d = torch_cuda.device("cuda:0")
+    P = torch_cuda.empty(0, dtype=torch_cuda.float64, device=d)
+    Y = torch_cuda.empty(m.n_species, 0, dtype=torch_cuda.float64, device=d)
+    W = torch_cuda.empty_like(Y)
+    dm.getrates_(P, P.clone(), Y, W)
+    torch_cuda.cuda.synchronize()
