@@ -14,6 +14,16 @@ import argparse
 import os
 
 
+def _warm(launch, seconds):
+    import time
+    import torch
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(4):
+            launch()
+        torch.cuda.synchronize()
+
+
 def main(argv=None):
     import numpy as np
     import torch
@@ -29,6 +39,8 @@ def main(argv=None):
     ap.add_argument("--tlo", type=float, default=800.0)
     ap.add_argument("--thi", type=float, default=2800.0)
     ap.add_argument("--weights", default=None, help="surrogate weights (.npz); default: the bundled ones")
+    ap.add_argument("--warm", type=float, default=0.0,
+                    help="seconds of untimed launches before the --reps ones (clocks settle under load)")
     a = ap.parse_args(argv)
     m = load_bundled(a.mech)
     if a.what == "batched":
@@ -39,6 +51,7 @@ def main(argv=None):
         P, T, Y = random_states(m.n_species, int(offs[-1]), seed=3, T_range=(a.tlo, a.thi))
         W = torch.empty_like(Y)
         dm.batched_prepare(offs)
+        _warm(lambda: dm.getrates_batched_prepared_(P, T, Y, W, grid=grid), a.warm)
         for _ in range(a.reps):
             dm.getrates_batched_prepared_(P, T, Y, W, grid=grid)
         tag = f"{v.tag} grid={grid}"
@@ -46,6 +59,7 @@ def main(argv=None):
         dm = DeviceMechanism(m, best_variant(m), build=False)
         P, T, Y = random_states(m.n_species, a.ng, seed=3, T_range=(a.tlo, a.thi))
         W = torch.empty_like(Y)
+        _warm(lambda: dm.getrates_(P, T, Y, W), a.warm)
         for _ in range(a.reps):
             dm.getrates_(P, T, Y, W)
         tag = dm.variant.tag
@@ -58,6 +72,7 @@ def main(argv=None):
         sur.load_state_dict(d)
         P, T, Y = random_states(m.n_species, a.ng, seed=7, T_range=(a.tlo, a.thi))
         W = torch.empty_like(Y)
+        _warm(lambda: sur.predict_(P, T, Y, W, fused=True), a.warm)
         for _ in range(a.reps):
             sur.predict_(P, T, Y, W, fused=True)
         tag = f"MLP {dims}"
