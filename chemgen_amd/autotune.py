@@ -55,6 +55,10 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
         # lane per point with reactions in live-range order, coefficients read or compiled in
         out += [Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True),
                 Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, True, True)]
+        if ns <= 24:   # small state: lane per point at higher occupancy
+            for wpe in (2, 4, 8):
+                out += [Variant(64, wpe, "lean", "gh", "horner", False, 4, 1, 0, False, True),
+                        Variant(64, wpe, "lean", "gh", "horner", False, 4, 1, 0, True, True)]
     if table_supported(mech):
         for npt in (16, 32, 64):
             for waves, wpe in ((4, 1), (8, 2), (16, 4)):
