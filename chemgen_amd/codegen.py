@@ -801,6 +801,16 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+def variant_flags(variant: "Variant") -> List[str]:
+    """Extra compiler flags of a variant.  Straight-line kernels turn off machine-level
+    loop-invariant code motion: in the persistent ragged-batch kernel it hoists the
+    materialised FP64 constants of the whole tile body out of the tile loop, which pushes
+    the kernel to the SGPR limit (40 spills, twice the LGKM waits of the loop-free body);
+    the kernels without a loop are unchanged by it.  The table kernel's reaction loops
+    keep it."""
+    return [] if variant.table else ["-mllvm", "-disable-machine-licm"]
+
+
 def compile_code_object(src_path: str, out_path: str, extra: List[str] = ()) -> None:
     """hipcc --genco: a gfx950 code object loadable with hipModuleLoad."""
     cmd = [HIPCC, "--genco", *HIP_FLAGS, *extra, "-o", out_path + ".tmp", src_path]

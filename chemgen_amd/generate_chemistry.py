@@ -120,7 +120,7 @@ def generate_getrates(args) -> None:
         raise SystemExit("this generator emits --language hip; the python/fortran targets are the "
                          "reference's own")
     from .api import save_generated
-    from .codegen import compile_code_object, emit_hip
+    from .codegen import compile_code_object, emit_hip, variant_flags
     mech = _load(args)
     v = variant_for(args, mech)
     stem = os.path.splitext(args.output)[0]
@@ -128,7 +128,7 @@ def generate_getrates(args) -> None:
     src, _, _ = emit_hip(mech, v)
     with open(stem + ".hip", "w") as fh:
         fh.write(src)
-    compile_code_object(stem + ".hip", stem + ".hsaco")
+    compile_code_object(stem + ".hip", stem + ".hsaco", variant_flags(v))
     save_generated(stem, mech, v, stem + ".hsaco")
     base = os.path.basename(stem)
     with open(stem + ".py", "w") as fh:

@@ -14,7 +14,7 @@ from typing import Optional
 
 import numpy as np
 
-from .codegen import Variant, build_coef, compile_code_object, emit_hip, structure_hash
+from .codegen import Variant, build_coef, compile_code_object, emit_hip, structure_hash, variant_flags
 from .mechanism import Mechanism
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -87,7 +87,7 @@ def ensure_code_object(mech: Mechanism, variant: Variant = Variant(), build: boo
     src_path = path[:-len(".hsaco")] + ".hip"
     with open(src_path, "w") as fh:
         fh.write(src)
-    compile_code_object(src_path, path)
+    compile_code_object(src_path, path, variant_flags(variant))
     return path
 
 
