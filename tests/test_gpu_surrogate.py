@@ -200,8 +200,9 @@ def test_fused_inference_matches_oracle(torch, name, hidden, layers, ng):
     m = load_bundled(name)
     dm = DeviceMechanism(m, build=False)
     sur = Surrogate(m, hidden=hidden, n_hidden=layers, seed=7)
+    g = torch.Generator().manual_seed(5)
     for L in sur.layers:          # biases away from zero, so the bias path is exercised
-        L.b.copy_(torch.randn(L.b.shape, device=L.b.device) * 0.3)
+        L.b.copy_((torch.randn(L.b.shape, generator=g) * 0.3).to(L.b.device))
     P, T, Y0 = random_states(m.n_species, ng, seed=31)
     sur.fit_scales(dm.getrates(P, T, Y0))
     Y = torch.zeros(m.n_species, ng + 37, dtype=torch.float64, device="cuda:0")[:, :ng]
@@ -220,9 +221,13 @@ def test_fused_inference_matches_oracle(torch, name, hidden, layers, ng):
     Wn = W.cpu().numpy()
     z = np.arcsinh(Wn / s[:, None]).T
     assert np.all(np.isfinite(z))
-    assert np.max(np.abs(z - z_ref) / (1.0 + np.abs(z_ref))) <= 5e-3
+    # BF16 activations: where the kernel's FP32 sums and the oracle's FP64 sums round an
+    # activation to neighbouring BF16 values, an output moves by up to ~2^-8 of its size
+    # per layer; a wrong fragment layout or permutation moves it by O(1)
+    tol = 2.0 ** -8 * (layers + 1)
+    assert np.max(np.abs(z - z_ref) / (1.0 + np.abs(z_ref))) <= tol
     zl = np.arcsinh(Wl.cpu().numpy() / s[:, None]).T
-    assert np.max(np.abs(z - zl) / (1.0 + np.abs(zl))) <= 5e-3
+    assert np.max(np.abs(z - zl) / (1.0 + np.abs(zl))) <= tol
     # residual and flag of the kernel's own wdot as the restated decode defines them (the
     # kernel's sinh and residual sums are single precision: ~1e-7 relative)
     w_chk, r_chk = np_decode(z, s, sur.aek.cpu().numpy())
