@@ -137,6 +137,8 @@ def structure_key(mech: Mechanism) -> str:
     """Everything the compiled code depends on (not the coefficient values)."""
     idx = {s.name: i for i, s in enumerate(mech.species)}
     parts = [f"ns={mech.n_species}", "tmid=" + ",".join(repr(s.thermo.t_mid) for s in mech.species)]
+    if getattr(mech, "qss", ()):
+        parts.append("qss=" + ",".join(str(idx[s]) for s in mech.qss))
     for r in mech.reactions:
         parts.append(repr((
             r.rtype, r.reversible,
@@ -218,6 +220,60 @@ def _pfac_pow(dnu: float) -> str:
     return f" * cg_exp2({dlit(-dnu)} * lg2pfac)"
 
 
+def _qss_terms(r: Reaction, idx: Dict[str, int], qpos: Dict[str, int], stage: int) -> List[str]:
+    """After kf (and M): F = kf prod_{transported reactants} C, B = kb prod_{transported
+    products} C (times M for three-body), with C = g h and kb = kf prod(g_r) / prod(g_p)
+    pfac^-dnu, so B = kf prod(g_r) pfac^-dnu prod_{transported products} h / g_b.  Stage 1
+    adds the reaction's terms to the QSS balance rows qA<i>_<k>, qc<i>; stage 2 forms
+    q = F x_a - B x_b and updates wdot of the transported species."""
+    L: List[str] = []
+    a = next((s for s in r.reactants if s in qpos), None)
+    b = next((s for s in r.products if s in qpos), None)
+    fr = [power_expr(f"(g{idx[s]} * h{idx[s]})", n) for s, n in r.reactants.items() if s not in qpos]
+    F = "kf" + "".join(f" * {t}" for t in fr)
+    if r.rtype == "third_body":
+        F = f"M * ({F})"
+    L.append(f"    const double qF = {F};")
+    rev = r.reversible
+    if rev:
+        gr = stoich_product(power_expr(f"g{idx[s]}", n) for s, n in r.reactants.items())
+        hp = [power_expr(f"h{idx[s]}", n) for s, n in r.products.items() if s not in qpos]
+        pf = _pf_factor(r.coeff_sum)
+        B = f"(kf * {gr})" + (f" * {pf}" if pf else "") + "".join(f" * {t}" for t in hp)
+        if b is not None:
+            B += f" * ig{idx[b]}"
+        if r.rtype == "third_body":
+            B = f"M * ({B})"
+        L.append(f"    const double qB = {B};")
+    if stage == 1:
+        if a is not None and a == b:
+            return L
+        if a is not None:
+            ia = qpos[a]
+            L.append(f"    qA{ia}_{ia} -= qF;")
+            if rev:
+                L.append(f"    qA{ia}_{qpos[b]} += qB;" if b is not None else f"    qc{ia} += qB;")
+        if b is not None:
+            ib = qpos[b]
+            if rev:
+                L.append(f"    qA{ib}_{ib} -= qB;")
+            L.append(f"    qA{ib}_{qpos[a]} += qF;" if a is not None else f"    qc{ib} += qF;")
+        return L
+    fq = f"qF * x{qpos[a]}" if a is not None else "qF"
+    if rev:
+        bq = f"qB * x{qpos[b]}" if b is not None else "qB"
+        L.append(f"    const double q = {fq} - {bq};")
+    else:
+        L.append(f"    const double q = {fq};")
+    for s_, n in r.reactants.items():
+        if s_ not in qpos:
+            L.append(f"    w{idx[s_]} -= {'q' if n == 1.0 else dlit(n) + ' * q'};")
+    for s_, n in r.products.items():
+        if s_ not in qpos:
+            L.append(f"    w{idx[s_]} += {'q' if n == 1.0 else dlit(n) + ' * q'};")
+    return L
+
+
 def _kr_eg(r: Reaction, idx: Dict[str, int], kf: str) -> str:
     num = stoich_product(power_expr(f"g{idx[s]}", n) for s, n in r.reactants.items())
     den = stoich_product(power_expr(f"g{idx[s]}", n) for s, n in r.products.items())
@@ -267,8 +323,13 @@ def _plog_code(r: Reaction, off: int) -> List[str]:
     return L
 
 
-def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp") -> List[str]:
-    L = [f"  {{  // R{j}: {_eq(r)}"]
+def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp",
+                   skip_eff=frozenset(), qss=None) -> List[str]:
+    """Straight-line code of one reaction: kf (and M), q and the wdot updates.  ``skip_eff``:
+    species left out of the third-body sum.  ``qss`` = (QSS species -> unknown index, stage):
+    stage 1 emits the reaction's terms of the QSS balance, stage 2 its rate with the solved
+    QSS concentrations x<i> (see chemgen_amd/qssa.py)."""
+    L = [f"  {{  // R{j}: {_eq(r)}" + (f" [QSS stage {qss[1]}]" if qss else "")]
     if kr_mode == "gh":
         cvar = lambda k: f"(g{k} * h{k})"
     else:
@@ -312,6 +373,8 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
         # M = ctot + sum (eff-1) C, in the order the efficiencies are listed
         M = "ctot"
         for q, s in enumerate(r.third_body):
+            if s in skip_eff:
+                continue
             M = f"__builtin_fma(cf[{tb_off + q}], {cvar(idx[s])}, {M})"
         L.append(f"    const double M = {M};")
     if r.rtype in ("troe", "lindemann", "sri"):
@@ -349,6 +412,10 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
                 L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
             else:
                 L.append(f"    const double kr = kf * cg_exp2(-({dg}));")
+    if qss is not None:
+        L.extend(_qss_terms(r, idx, qss[0], qss[1]))
+        L.append("  }")
+        return L
     if kr_mode == "gh":
         # q = kf prod(C_r) - kb prod(C_p) with C = g h and kb = kf prod(g_r)/prod(g_p) pfac^-dnu:
         # q = kf prod(g_r) (prod(h_r) - pfac^-dnu prod(h_p))
@@ -473,6 +540,8 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
         return _emit_table(mech, variant)
     if variant.split > 1:
         return _emit_split(mech, variant)
+    if getattr(mech, "qss", ()):
+        return _emit_qss(mech, variant)
     if variant.live:
         return _emit_live(mech, variant)
     import jinja2
@@ -602,6 +671,135 @@ def _emit_live(mech: Mechanism, variant: Variant):
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h, peak_live=peak)
+    return src, coef, meta
+
+
+def _emit_qss(mech, variant: Variant):
+    """Lane-per-point kernel of a mechanism with QSS species (chemgen_amd/qssa.py): Y and
+    wdot hold the transported species only; the QSS concentrations come from the linear QSS
+    balance, assembled in registers (stage 1 over the reactions that involve a QSS species),
+    solved by Gaussian elimination without pivoting, then used by stage 2."""
+    import jinja2
+    if variant.split != 1 or variant.table or variant.g_lds or variant.live:
+        raise ValueError("QSS species: lane-per-point kernels only")
+    coef, off_smh, roffs = build_coef(mech)
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    qset = set(mech.qss)
+    qpos = {s: i for i, s in enumerate(mech.qss)}
+    nq = len(mech.qss)
+    R = [k for k, s in enumerate(mech.species) if s.name not in qset]
+    row = {k: r for r, k in enumerate(R)}
+    tmids, group_of = [], []
+    for sp in mech.species:
+        if sp.thermo.t_mid not in tmids:
+            tmids.append(sp.thermo.t_mid)
+        group_of.append(tmids.index(sp.thermo.t_mid))
+    involved = [j for j, r in enumerate(mech.reactions)
+                if any(s in qset for s in list(r.reactants) + list(r.products))]
+    # every QSS species must be consumed somewhere, or its balance row has no diagonal
+    consumed = set()
+    for j in involved:
+        r = mech.reactions[j]
+        a = next((s for s in r.reactants if s in qset), None)
+        b = next((s for s in r.products if s in qset), None)
+        if a is not None and a != b:
+            consumed.add(a)
+        if b is not None and a != b and r.reversible:
+            consumed.add(b)
+    if consumed != qset:
+        raise ValueError(f"QSS species never consumed: {sorted(qset - consumed)}")
+    L: List[str] = []
+    for k, sp in enumerate(mech.species):
+        L += [f"  // species {k} ({sp.name}){' [QSS]' if sp.name in qset else ''}",
+              f"  double g{k};", "  {",
+              f"    const auto* a = cf + {off_smh + 14 * k};",
+              "    double hl = a[6] * invT, hh = a[13] * invT;",
+              "    hl = __builtin_fma(a[1], lg2T, hl); hh = __builtin_fma(a[8], lg2T, hh);",
+              "    hl = __builtin_fma(a[2], t, hl);   hh = __builtin_fma(a[9], t, hh);",
+              "    hl = __builtin_fma(a[3], t2, hl);  hh = __builtin_fma(a[10], t2, hh);",
+              "    hl = __builtin_fma(a[4], t3, hl);  hh = __builtin_fma(a[11], t3, hh);",
+              "    hl = __builtin_fma(a[5], t4, hl);  hh = __builtin_fma(a[12], t4, hh);",
+              "    hl = hl + a[0];                    hh = hh + a[7];",
+              f"    g{k} = cg_exp2_fin(lo{group_of[k]} ? hl : hh);", "  }"]
+        if sp.name in qset:
+            L.append(f"  const double ig{k} = cg_div(1.0, g{k});")
+        else:
+            L += [f"  const double h{k} = cg_div(Y[{row[k]} * ld + i] * cf[{k}] * scale, g{k});",
+                  f"  double w{k} = 0.0;"]
+    for i in range(nq):
+        L.append("  double " + ", ".join(f"qA{i}_{k} = 0.0" for k in range(nq)) + f", qc{i} = 0.0;")
+    for j, r in enumerate(mech.reactions):
+        if j in involved:
+            L.extend(_reaction_code(j, r, roffs[j], idx, "gh", qset, (qpos, 1)))
+        else:
+            L.extend(_reaction_code(j, r, roffs[j], idx, "gh", qset))
+    # A x = -c by Gaussian elimination (no pivoting), fill-in tracked structurally
+    nz = set()
+    for j in involved:
+        r = mech.reactions[j]
+        a = next((s for s in r.reactants if s in qset), None)
+        b = next((s for s in r.products if s in qset), None)
+        if a is not None and a == b:
+            continue
+        if a is not None:
+            nz.add((qpos[a], qpos[a]))
+            if b is not None and r.reversible:
+                nz.add((qpos[a], qpos[b]))
+        if b is not None:
+            if r.reversible:
+                nz.add((qpos[b], qpos[b]))
+            if a is not None:
+                nz.add((qpos[b], qpos[a]))
+    L.append("  // QSS balance A x = -c: forward elimination")
+    for p in range(nq):
+        L.append(f"  const double qp{p} = cg_div(1.0, qA{p}_{p});")
+        for i in range(p + 1, nq):
+            if (i, p) not in nz:
+                continue
+            L.append(f"  {{ const double f = qA{i}_{p} * qp{p};")
+            for k in range(p + 1, nq):
+                if (p, k) in nz:
+                    L.append(f"    qA{i}_{k} = __builtin_fma(-f, qA{p}_{k}, qA{i}_{k});")
+                    nz.add((i, k))
+            L.append(f"    qc{i} = __builtin_fma(-f, qc{p}, qc{i}); }}")
+    L.append("  // back substitution: the QSS concentrations")
+    for p in reversed(range(nq)):
+        expr = f"-qc{p}"
+        for k in range(p + 1, nq):
+            if (p, k) in nz:
+                expr = f"__builtin_fma(-qA{p}_{k}, x{k}, {expr})"
+        L.append(f"  const double x{p} = ({expr}) * qp{p};")
+    # stage 2 recomputes the rates of the QSS reactions from fresh (opaque) copies of the
+    # point's inputs, so the compiler does not carry every stage-1 rate across the solve
+    names = ("t", "lg2T", "invT", "ctot", "pfac", "pinv", "lg2pfac")
+    L.append("  {")
+    L.append("    double " + ", ".join(f"o_{n} = {n}" for n in names) + ";")
+    L.append('    __asm__ volatile("" : ' + ", ".join(f'"+v"(o_{n})' for n in names) + ");")
+    L.append("    const double " + ", ".join(f"{n} = o_{n}" for n in names) + ";")
+    L.append("    " + " ".join(f"(void){n};" for n in names))
+    for j in involved:
+        L.extend("  " + ln for ln in _reaction_code(j, mech.reactions[j], roffs[j], idx, "gh", qset, (qpos, 2)))
+    L.append("  }")
+    for k in R:
+        L.append(f"  W[{row[k]} * ld + i] = w{k};")
+    env = jinja2.Environment(loader=jinja2.FileSystemLoader(TEMPLATE_DIR), keep_trailing_newline=True)
+    tpl = env.get_template("getrates_live_hip.j2")
+    h = structure_hash(mech)
+    note = (f"// FP64 getrates for gfx950 with {nq} quasi-steady-state species ({', '.join(mech.qss)}):\n"
+            f"// one lane per grid point; Y and wdot hold the {len(R)} transported species (rows in\n"
+            f"// mechanism order without the QSS species); the QSS concentrations solve the linear QSS\n"
+            f"// balance of the {len(involved)} reactions that involve them.")
+    src = tpl.render(
+        name=mech.name, ns=len(R), nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
+        waves_per_eu=variant.waves_per_eu, hash_hex=f"0x{h:016x}", peak=0, note=note,
+        csum_expr=tree_sum([f"Y[{row[k]} * ld + i] * cf[{k}]" for k in R]),
+        R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)),
+        patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, poly=variant.poly,
+        tmid_groups=[{"tmid": dlit(t)} for t in tmids], program_code="\n".join(L),
+        threads=variant.threads, ppb=variant.ppb, **_baked_args(coef, variant),
+    )
+    meta = dict(ns=len(R), nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
+                waves_per_eu=variant.waves_per_eu, structure_hash=h, n_qss=nq, qss_reactions=len(involved))
     return src, coef, meta
 
 
