@@ -237,6 +237,45 @@ def bench_h2_miniapp(steps, warmup, ngg=1 << 21, ng_cpu=1 << 18):
             "parity": {"sample_points": nchk, **par}}
 
 
+def bench_qss(steps, warmup, ng=1 << 20):
+    """GRI-Mech 3.0 with 12 quasi-steady-state species (chemgen_amd/qssa.py): getrates over
+    the 41 transported species of 2^20 points, T U(800, 2500) K, seed 0, with the QSS kernel
+    whose coefficients are compiled in; parity against the QSS oracle on a sample."""
+    import numpy as np
+    import torch
+    from chemgen_amd.codegen import Variant
+    from chemgen_amd.qssa import bundled_qss
+    from chemgen_amd.runtime import DeviceMechanism
+    from chemgen_amd.synth import make_states
+    from oracle.qssa_np import qss_gross_scale, qss_getrates
+    qm = bundled_qss("gri30")
+    v = Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, True)
+    P, T, Y = make_states(qm.n_transported, ng, seed=0, T_range=(800.0, 2500.0))
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(x).to(dev) for x in (P, T, Y))
+    Wd = torch.empty_like(Yd)
+    dm = DeviceMechanism(qm, v, build=False)
+    s = torch.cuda.current_stream()
+    warm_up(lambda: dm.getrates_(Pd, Td, Yd, Wd, s), warmup)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        dm.getrates_(Pd, Td, Yd, Wd, s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / steps / 1e3
+    nchk = 1024
+    ref, _, qf, qb, _ = qss_getrates(qm, qm.qss, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
+    G = qss_gross_scale(qm, qm.qss, qf, qb)
+    d = np.abs(Wd[:, :nchk].cpu().numpy() - ref)
+    live = G > 0
+    return {"workload": f"gri30 with {len(qm.qss)} QSS species ({', '.join(qm.qss)}): {qm.n_transported} "
+                        f"transported species, ng={ng}, T U(800,2500) K, P 1-40 atm, Y Dirichlet(0.5), seed 0",
+            "points_per_s": ng / t, "kernel_avg_ms": t * 1e3, "kernel_variant": v.tag,
+            "parity": {"sample_points": nchk, "max_gross": float(np.max(d[live] / (np.abs(ref) + G)[live])),
+                       "inert_exact": bool(np.all(d[~live] == 0.0))}}
+
+
 def bench_surrogate(steps, warmup, ng=1 << 20):
     """The learned surrogate for the GRI-sized mechanism (4 x 256 SiLU MLP, BF16 on the matrix
     cores, weights in chemgen_amd/data) on 2^20 held-out states (T U(800, 2800) K, seed 7):
@@ -322,6 +361,7 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate workload")
     ap.add_argument("--no-h2", action="store_true", help="skip the H2 Burke mini-app workload")
+    ap.add_argument("--no-qss", action="store_true", help="skip the QSS-species workload")
     ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
                     help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
@@ -423,6 +463,9 @@ def main():
     if not args.no_h2:
         torch.cuda.empty_cache()
         out["h2_miniapp"] = bench_h2_miniapp(args.steps, args.warmup)
+    if not args.no_qss:
+        torch.cuda.empty_cache()
+        out["qss"] = bench_qss(args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
     print(json.dumps(_finite(out)))

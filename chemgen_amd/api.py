@@ -40,12 +40,14 @@ class GetRates:
             from .autotune import best_variant
             variant = best_variant(mech)
         self.mech = mech
-        self.species: List[str] = mech.species_names
+        q = set(getattr(mech, "qss", ()))
+        self.species: List[str] = [n for n in mech.species_names if n not in q]
         self._dm = DeviceMechanism(mech, variant, build=build, coef=coef, code_object=code_object)
 
     @property
     def n_species(self) -> int:
-        return self.mech.n_species
+        """Species in Y and wdot (the transported ones when the mechanism has QSS species)."""
+        return getattr(self.mech, "n_transported", self.mech.n_species)
 
     def __call__(self, P, T, Y, layout: Optional[str] = None):
         import torch
@@ -130,6 +132,7 @@ def save_generated(stem: str, mech: Mechanism, variant: Variant, code_object: st
     meta = {"mechanism": mech.name, "species": mech.species_names,
             "mw": [s.mw for s in mech.species], "structure_hash": f"{structure_hash(mech):016x}",
             "variant": __import__("dataclasses").asdict(variant),
+            "qss": list(getattr(mech, "qss", ())),
             "code_object": os.path.basename(code_object)}
     with open(stem + ".json", "w") as fh:
         json.dump(meta, fh, indent=1)
@@ -140,6 +143,9 @@ def load_generated(stem: str) -> GetRates:
     with open(stem + ".json") as fh:
         meta = json.load(fh)
     mech = load_mechanism(stem + ".inp", name=meta["mechanism"])
+    if meta.get("qss"):
+        from .qssa import with_qss
+        mech = with_qss(mech, meta["qss"], name=meta["mechanism"])
     if f"{structure_hash(mech):016x}" != meta["structure_hash"]:
         raise ChemgenError(f"{stem}.inp does not match the generated code object")
     coef = np.load(stem + ".coef.npy")

@@ -64,6 +64,10 @@ def parse_arguments(argv=None):
     p.add_argument("--time-cpu", action="store_true")
     p.add_argument("--compat", choices=["full", "reference"], default="full",
                    help="reference: keep only the reaction types the reference handles")
+    p.add_argument("--qssa", default="",
+                   help="comma-separated quasi-steady-state species (the reference's "
+                        "chemistry(qssa_species=...)): Y and wdot then hold the other species "
+                        "only, the QSS concentrations come from the linear QSS balance")
     p.add_argument("--legacy-abi", action="store_true",
                    help="also build lib<stem>_legacy.so exporting the reference's C entry points "
                         "(copyConstantsToDevice, getrates_hip_wrapper_v1..v4) over this kernel")
@@ -73,6 +77,8 @@ def parse_arguments(argv=None):
 def variant_for(args, mech):
     from .autotune import best_variant
     from .codegen import Variant
+    if getattr(mech, "qss", ()):
+        return Variant()          # QSS kernels: lane per point
     lvl = args.parallel_level
     if lvl == 1:
         return Variant(block=max(64, (args.veclen + 63) // 64 * 64), waves_per_eu=1)
@@ -91,7 +97,12 @@ def variant_for(args, mech):
 
 def _load(args):
     from .mechanism import load_mechanism
-    return load_mechanism(args.mech, args.therm, args.yaml_file, compat=args.compat)
+    mech = load_mechanism(args.mech, args.therm, args.yaml_file, compat=args.compat)
+    qss = [s.strip() for s in getattr(args, "qssa", "").split(",") if s.strip()]
+    if qss:
+        from .qssa import with_qss
+        mech = with_qss(mech, qss)
+    return mech
 
 
 def generate_gpu_coefficients(args) -> None:

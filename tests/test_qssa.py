@@ -119,3 +119,26 @@ def test_qss_kernel_source_structure():
     assert all(f"const double x{i} =" in src for i in range(len(GRI_QSS)))
     assert f"W[{meta['ns'] - 1} * ld + i]" in src and f"W[{meta['ns']} * ld + i]" not in src
     assert structure_hash(qm) != structure_hash(g)
+
+
+def test_generator_writes_a_qss_module(tmp_path):
+    """generate_chemistry --qssa: the generated kernel source and the saved metadata carry the
+    QSS set, and the module's species list is the transported one."""
+    import json
+    import os
+    from chemgen_amd import generate_chemistry
+    from chemgen_amd.kernels import DATA_DIR
+    out = tmp_path / "h2q.py"
+    args = generate_chemistry.parse_arguments(["--mode", "getrates", "--language", "hip", "--mech",
+                                               os.path.join(DATA_DIR, "h2_burke.inp"), "--output", str(out),
+                                               "--qssa", "H2O2"])
+    mech = generate_chemistry._load(args)
+    assert tuple(mech.qss) == ("H2O2",) and mech.n_transported == 8
+    from chemgen_amd.api import save_generated
+    from chemgen_amd.codegen import emit_hip
+    v = generate_chemistry.variant_for(args, mech)
+    src, _, meta = emit_hip(mech, v)
+    assert meta["ns"] == 8 and "quasi-steady-state" in src
+    stem = str(tmp_path / "h2q")
+    save_generated(stem, mech, v, stem + ".hsaco")
+    assert json.load(open(stem + ".json"))["qss"] == ["H2O2"]
