@@ -33,6 +33,32 @@ def warm_up(launch, warmup, seconds=WARM_S):
     return n
 
 
+def measured_fp64_peak(iters=20000):
+    """FP64 FMA ceiling of this GPU (cg_probe_fp64_fma: eight independent FMA chains per lane,
+    8 waves per SIMD), TFLOP/s, best of five timed launches after a warm-up."""
+    import ctypes
+    import torch
+    from chemgen_amd.runtime import lib
+    L = lib()
+    L.cg_probe_fp64_fma.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+    L.cg_probe_fp64_fma.restype = ctypes.c_int
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    blocks = n_cu * 8
+    out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda:0")
+    s = torch.cuda.current_stream()
+    launch = lambda: L.cg_probe_fp64_fma(out.data_ptr(), blocks, iters, ctypes.c_void_p(s.cuda_stream))
+    warm_up(launch, 2, 0.5)
+    best = 0.0
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        launch()
+        e1.record(s)
+        torch.cuda.synchronize()
+        best = max(best, 2.0 * 8 * iters * 256 * blocks / (e0.elapsed_time(e1) / 1e3))
+    return best / 1e12
+
+
 def cpu_baseline(mech_name, mech, target_s=10.0, sample_ng=1 << 14):
     """C/OpenMP oracle on the host cores over a bounded sample of the same states."""
     import numpy as np
@@ -419,6 +445,7 @@ def main():
 
     w = work_per_point(mech)
     achieved = w["flops"] * ng / avg_launch_s
+    peak_meas = measured_fp64_peak()
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
@@ -446,8 +473,10 @@ def main():
                    "mechanism": args.mech, "ng": ng, "kernel_variant": dm.variant.tag},
         "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": PEAK_FP64_VECTOR / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VECTOR, "traffic": traffic,
-                     "flops_per_point": w["flops"], "bytes_per_point": w["bytes"],
-                     "kernel_avg_ms": avg_launch_s * 1e3},
+                     "flops_per_point": w["flops"], "flops_arith_per_point": w["arith"],
+                     "transcendentals_per_point": w["transc"], "bytes_per_point": w["bytes"],
+                     "kernel_avg_ms": avg_launch_s * 1e3,
+                     "measured_peak": peak_meas, "frac_of_measured": achieved / 1e12 / peak_meas},
         "parity": {"sample_points": nchk, **par},
     }
     if not args.no_batched:

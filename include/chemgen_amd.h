@@ -143,6 +143,13 @@ int cg_mlp_infer(const double* P, const double* T, const double* Y, int64_t ld, 
                  int32_t n_hidden, const double* s, const float* aek, int32_t ne, double tol, double* W,
                  int64_t ldw, int32_t* flag, double* resid, void* stream);
 
+/* ---- measurement (no counterpart in the reference) ---- */
+
+/* FP64 FMA throughput probe for the roofline's measured ceiling: `blocks` workgroups of 256
+ * lanes, each lane eight independent FMA chains of `iters` steps (2 * 8 * iters * 256 * blocks
+ * flops), one result per lane into `out` (device, blocks * 256 doubles). */
+int cg_probe_fp64_fma(double* out, int32_t blocks, int32_t iters, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

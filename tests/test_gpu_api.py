@@ -147,3 +147,26 @@ def test_batched_graph_replay():
         torch.cuda.synchronize()
         assert torch.equal(W, eager), f"replay {rep}"
         _ = (Yd * 2.0).sum().item()  # eager work between replays
+
+
+def test_fp64_probe_values():
+    """The FMA probe behind the bench's measured FP64 ceiling computes what it claims: eight
+    chains x <- a x + b per lane, x0 = lane/1000 + c for chain c."""
+    import ctypes
+    import torch
+    from chemgen_amd.runtime import lib
+    L = lib()
+    L.cg_probe_fp64_fma.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+    L.cg_probe_fp64_fma.restype = ctypes.c_int
+    blocks, iters = 3, 50
+    out = torch.zeros(blocks * 256, dtype=torch.float64, device="cuda:0")
+    assert L.cg_probe_fp64_fma(out.data_ptr(), blocks, iters, None) == 0
+    torch.cuda.synchronize()
+    a, b = 0.999999, 1e-7
+    lane = np.tile(np.arange(256), blocks) * 1e-3
+    x = np.stack([lane + c for c in range(8)])
+    for _ in range(iters):
+        x = a * x + b
+    ref = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]))
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-13)
+    assert L.cg_probe_fp64_fma(out.data_ptr(), 0, iters, None) != 0
