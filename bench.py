@@ -381,7 +381,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mech", default="synth_gri53")
     ap.add_argument("--ng", type=int, default=1 << 20)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
     ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized workload")
