@@ -16,6 +16,27 @@ import json
 import time
 
 
+def bf16_floor(sur, P, T, Y, W, n: int = 4096) -> dict:
+    """How far BF16 arithmetic alone moves this network's outputs: the network's fused BF16
+    evaluation W against the same weights (FP32 masters) evaluated in FP64 on the same BF16
+    features, per point max_k |dw| / (|w| + s_k).  A network whose FP64 evaluation were exact
+    could still be within 1% only on the points this leaves within 1%."""
+    import numpy as np
+    n = min(n, P.shape[0])
+    X = sur.features(P[:n], T[:n], Y[:, :n].contiguous()).double().cpu().numpy()
+    h = X
+    for l, L in enumerate(sur.layers):
+        h = h @ L.w.double().cpu().numpy().T + L.b.double().cpu().numpy()
+        if l < len(sur.layers) - 1:
+            h = h / (1.0 + np.exp(-h))
+    s = sur.s.cpu().numpy()
+    w64 = s[None, :] * np.sinh(h)
+    wq = W[:, :n].double().cpu().numpy().T
+    e = np.max(np.abs(wq - w64) / (np.abs(w64) + s[None, :]), axis=1)
+    return {"points": n, "frac_le_1pct": float(np.mean(e <= 1e-2)), "median": float(np.median(e)),
+            "p90": float(np.percentile(e, 90))}
+
+
 def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: int = 64, reps: int = 5):
     """Accuracy (before / after the fallback) and throughput on n_eval held-out states."""
     import torch
@@ -33,6 +54,7 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
     W = torch.empty_like(Y)
     flag, resid = sur.predict_(P, T, Y, W, tol)
     err_raw = sur.error(W, ref)
+    floor = bf16_floor(sur, P, T, Y, W)
     info = sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every)
     err = sur.error(W, ref)
     torch.cuda.synchronize()
@@ -64,6 +86,7 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
                           "p99_err": q(err, 0.99), **info,
                           "fallback_frac": info["fallback_points"] / n_eval},
         "residual_flag_frac": float(flag.double().mean()),
+        "bf16_vs_fp64_same_weights": floor,
         "fp64_repeat": {"frac_le_1pct": float((err_rep <= 1e-2).double().mean()),
                         "median_err": q(err_rep, 0.5), "max_err": float(err_rep.max().item())},
         "fp64_kernel_points_per_s": n_eval / t_fp64,
