@@ -964,8 +964,9 @@ def _emit_table(mech: Mechanism, variant: Variant):
     if 64 % npt:
         raise ValueError("table kernel: points per tile must divide 64")
     slots = W * 64 // npt
-    # a class gets a compiled loop when it fills every reaction slot of a tile at least once
-    rows, effsp, order, segs, gen_start = table_records(mech, npt, min_class=slots)
+    # a class gets a compiled loop when it fills at least half the reaction slots of a tile
+    # (JetSurf-sized: 2.6% faster than a full round of slots, every class compiled 5% slower)
+    rows, effsp, order, segs, gen_start = table_records(mech, npt, min_class=max(1, slots // 2))
     classes = []
     for (a, b, (fam, rev, nrs, nps, dnu)) in segs:
         pf = None
