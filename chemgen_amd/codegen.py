@@ -677,8 +677,9 @@ def _emit_live(mech: Mechanism, variant: Variant):
 def _emit_qss(mech, variant: Variant):
     """Lane-per-point kernel of a mechanism with QSS species (chemgen_amd/qssa.py): Y and
     wdot hold the transported species only; the QSS concentrations come from the linear QSS
-    balance, assembled in registers (stage 1 over the reactions that involve a QSS species),
-    solved by Gaussian elimination without pivoting, then used by stage 2."""
+    balance, assembled in registers (stage 1 over the reactions that involve a QSS species)
+    and solved by Gaussian elimination without pivoting before any wdot accumulator is live;
+    then every reaction runs in mechanism order, those with QSS species as stage 2."""
     import jinja2
     if variant.split != 1 or variant.table or variant.g_lds or variant.live:
         raise ValueError("QSS species: lane-per-point kernels only")
@@ -724,15 +725,12 @@ def _emit_qss(mech, variant: Variant):
         if sp.name in qset:
             L.append(f"  const double ig{k} = cg_div(1.0, g{k});")
         else:
-            L += [f"  const double h{k} = cg_div(Y[{row[k]} * ld + i] * cf[{k}] * scale, g{k});",
-                  f"  double w{k} = 0.0;"]
+            L.append(f"  const double h{k} = cg_div(Y[{row[k]} * ld + i] * cf[{k}] * scale, g{k});")
+    # stage 1 (the QSS balance) and the solve come first, before any wdot accumulator is live
     for i in range(nq):
         L.append("  double " + ", ".join(f"qA{i}_{k} = 0.0" for k in range(nq)) + f", qc{i} = 0.0;")
-    for j, r in enumerate(mech.reactions):
-        if j in involved:
-            L.extend(_reaction_code(j, r, roffs[j], idx, "gh", qset, (qpos, 1)))
-        else:
-            L.extend(_reaction_code(j, r, roffs[j], idx, "gh", qset))
+    for j in involved:
+        L.extend(_reaction_code(j, mech.reactions[j], roffs[j], idx, "gh", qset, (qpos, 1)))
     # A x = -c by Gaussian elimination (no pivoting), fill-in tracked structurally
     nz = set()
     for j in involved:
@@ -772,13 +770,17 @@ def _emit_qss(mech, variant: Variant):
     # stage 2 recomputes the rates of the QSS reactions from fresh (opaque) copies of the
     # point's inputs, so the compiler does not carry every stage-1 rate across the solve
     names = ("t", "lg2T", "invT", "ctot", "pfac", "pinv", "lg2pfac")
+    L.append("  " + " ".join(f"double w{k} = 0.0;" for k in R))
     L.append("  {")
     L.append("    double " + ", ".join(f"o_{n} = {n}" for n in names) + ";")
     L.append('    __asm__ volatile("" : ' + ", ".join(f'"+v"(o_{n})' for n in names) + ");")
     L.append("    const double " + ", ".join(f"{n} = o_{n}" for n in names) + ";")
     L.append("    " + " ".join(f"(void){n};" for n in names))
-    for j in involved:
-        L.extend("  " + ln for ln in _reaction_code(j, mech.reactions[j], roffs[j], idx, "gh", qset, (qpos, 2)))
+    for j, r in enumerate(mech.reactions):
+        if j in involved:
+            L.extend("  " + ln for ln in _reaction_code(j, r, roffs[j], idx, "gh", qset, (qpos, 2)))
+        else:
+            L.extend("  " + ln for ln in _reaction_code(j, r, roffs[j], idx, "gh", qset))
     L.append("  }")
     for k in R:
         L.append(f"  W[{row[k]} * ld + i] = w{k};")
