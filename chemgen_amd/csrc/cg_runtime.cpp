@@ -65,6 +65,7 @@ struct cg_mech {
   int32_t* tile_len_d = nullptr;
   int64_t tile_cap = 0;
   int64_t n_tiles = 0;
+  int64_t n_points = 0;  // offsets[n_patches] of the prepared batch: the launch needs ld >= it
 };
 
 extern "C" const char* cg_last_error(void) { return g_err.c_str(); }
@@ -219,6 +220,7 @@ extern "C" int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n
     CG_HIP(hipMemcpy(h->tile_len_d, len.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
   }
   h->n_tiles = nt;
+  h->n_points = offsets[n_patches];
   return CG_OK;
 }
 
@@ -227,7 +229,9 @@ extern "C" int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const 
                                             void* stream) {
   if (!h) return fail(CG_EINVAL, "cg_getrates_batched_prepared: null handle");
   if (h->n_tiles == 0) return CG_OK;
-  if (!P || !T || !Y || !W || ld <= 0) return fail(CG_EINVAL, "cg_getrates_batched_prepared: bad array");
+  if (!P || !T || !Y || !W || ld < h->n_points)
+    return fail(CG_EINVAL, "cg_getrates_batched_prepared: bad array or ld < %lld points",
+                (long long)h->n_points);
   // grid > 0: that many persistent workgroups; 0: 8 per CU; < 0: one workgroup per tile
   int64_t g = grid > 0 ? grid : (grid < 0 ? h->n_tiles : (int64_t)h->n_cu * 8);
   if (g > h->n_tiles) g = h->n_tiles;
@@ -245,7 +249,7 @@ extern "C" int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const 
 extern "C" int cg_getrates_batched(cg_mech_t h, const double* P, const double* T, const double* Y,
                                    double* W, int64_t ld, const int64_t* offsets, int32_t n_patches,
                                    int32_t grid, void* stream) {
-  if (!h || !offsets) return fail(CG_EINVAL, "cg_getrates_batched: bad argument");
+  if (!h || !offsets || n_patches < 0) return fail(CG_EINVAL, "cg_getrates_batched: bad argument");
   if (offsets[n_patches] > ld) return fail(CG_EINVAL, "cg_getrates_batched: offsets exceed ld");
   int rc = cg_batched_prepare(h, offsets, n_patches);
   if (rc != CG_OK) return rc;

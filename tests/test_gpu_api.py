@@ -170,3 +170,23 @@ def test_fp64_probe_values():
     ref = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]))
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-13)
     assert L.cg_probe_fp64_fma(out.data_ptr(), 0, iters, None) != 0
+
+
+def test_batched_launch_checks_leading_dimension():
+    """A prepared batch refuses arrays whose leading dimension is shorter than the batch's
+    last point (the tiles would run past the species rows)."""
+    import torch
+    from chemgen_amd.runtime import ChemgenError, DeviceMechanism
+    m = load_bundled("h2_burke")
+    dm = DeviceMechanism(m, build=False)
+    offs = np.array([0, 100, 300], dtype=np.int64)
+    d = torch.device("cuda:0")
+    P = torch.ones(300, dtype=torch.float64, device=d) * 1013250.0
+    T = torch.ones(300, dtype=torch.float64, device=d) * 1200.0
+    Y = torch.full((m.n_species, 200), 1.0 / m.n_species, dtype=torch.float64, device=d)
+    W = torch.empty_like(Y)
+    dm.batched_prepare(offs)
+    with pytest.raises(ChemgenError):
+        dm.getrates_batched_prepared_(P, T, Y, W)
+    with pytest.raises(ChemgenError):
+        dm.getrates_batched_(P, T, Y, W, offs)
