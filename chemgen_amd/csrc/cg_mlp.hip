@@ -647,6 +647,8 @@ extern "C" int cg_mlp_features(const double* P, const double* T, const double* Y
 
 extern "C" int cg_mlp_targets(const double* W, int64_t ld, int64_t ng, int32_t ns, const double* s,
                               float* Z, void* stream) {
+  if (ng < 0 || ns <= 0 || ld < ng || (ng > 0 && (!W || !s || !Z)))
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_targets: bad argument");
   if (ng == 0) return CG_OK;
   targets_k<<<nblk(ng, 256), 256, 0, (hipStream_t)stream>>>(W, ld, ng, ns, s, Z);
   CG_LAUNCH_CHECK("cg_mlp_targets launch");
@@ -655,6 +657,7 @@ extern "C" int cg_mlp_targets(const double* W, int64_t ld, int64_t ng, int32_t n
 
 extern "C" int cg_mlp_mse_grad(const float* Zh, const float* Z, int64_t n, float scale, void* G,
                                float* loss, void* stream) {
+  if (n < 0 || (n > 0 && (!Zh || !Z || !G))) return cg_internal_fail(CG_EINVAL, "cg_mlp_mse_grad: bad argument");
   if (n == 0) return CG_OK;
   mse_grad_k<<<nblk(n, 256), 256, 0, (hipStream_t)stream>>>(Zh, Z, n, scale, (bf16*)G, loss);
   CG_LAUNCH_CHECK("cg_mlp_mse_grad launch");
@@ -662,6 +665,7 @@ extern "C" int cg_mlp_mse_grad(const float* Zh, const float* Z, int64_t n, float
 }
 
 extern "C" int cg_mlp_silu_back(const float* dH, const float* Zp, int64_t n, void* dZ, void* stream) {
+  if (n < 0 || (n > 0 && (!dH || !Zp || !dZ))) return cg_internal_fail(CG_EINVAL, "cg_mlp_silu_back: bad argument");
   if (n == 0) return CG_OK;
   silu_back_k<<<nblk(n, 256), 256, 0, (hipStream_t)stream>>>(dH, Zp, n, (bf16*)dZ);
   CG_LAUNCH_CHECK("cg_mlp_silu_back launch");
@@ -669,7 +673,8 @@ extern "C" int cg_mlp_silu_back(const float* dH, const float* Zp, int64_t n, voi
 }
 
 extern "C" int cg_mlp_colsum(const void* G, int64_t M, int32_t N, float* db, void* stream) {
-  if (M == 0) return CG_OK;
+  if (M < 0 || N < 0 || (M > 0 && N > 0 && (!G || !db))) return cg_internal_fail(CG_EINVAL, "cg_mlp_colsum: bad argument");
+  if (M == 0 || N == 0) return CG_OK;
   dim3 grid(nblk(N, 256), nblk(M, 1024));
   colsum_k<<<grid, 256, 0, (hipStream_t)stream>>>((const bf16*)G, M, N, db);
   CG_LAUNCH_CHECK("cg_mlp_colsum launch");
@@ -678,6 +683,8 @@ extern "C" int cg_mlp_colsum(const void* G, int64_t M, int32_t N, float* db, voi
 
 extern "C" int cg_mlp_adam(float* w, const float* g, float* m1, float* m2, int64_t n, float lr,
                            float b1, float b2, float eps, int32_t step, void* w16, void* stream) {
+  if (n < 0 || step < 1 || (n > 0 && (!w || !g || !m1 || !m2)))
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_adam: bad argument");
   if (n == 0) return CG_OK;
   const float c1 = 1.0f / (1.0f - powf(b1, (float)step));
   const float c2 = 1.0f / (1.0f - powf(b2, (float)step));
@@ -687,6 +694,7 @@ extern "C" int cg_mlp_adam(float* w, const float* g, float* m1, float* m2, int64
 }
 
 extern "C" int cg_mlp_to_bf16(const float* x, int64_t n, void* y, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !y))) return cg_internal_fail(CG_EINVAL, "cg_mlp_to_bf16: bad argument");
   if (n == 0) return CG_OK;
   to_bf16_k<<<nblk(n, 256), 256, 0, (hipStream_t)stream>>>(x, n, (bf16*)y);
   CG_LAUNCH_CHECK("cg_mlp_to_bf16 launch");
@@ -697,6 +705,8 @@ extern "C" int cg_mlp_decode(const float* Zh, int64_t ng, int32_t ns, const doub
                              int32_t ne, double tol, double* W, int64_t ld, int32_t* flag, double* resid,
                              void* stream) {
   if (ne > 8) return cg_internal_fail(CG_ERANGE, "cg_mlp_decode: at most 8 elements");
+  if (ng < 0 || ns <= 0 || ne < 0 || ld < ng || (ng > 0 && (!Zh || !s || !W || !flag || !resid || (ne > 0 && !aek))))
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_decode: bad argument");
   if (ng == 0) return CG_OK;
   decode_k<<<nblk(ng, 256), 256, 0, (hipStream_t)stream>>>(Zh, ng, ns, s, aek, ne, tol, W, ld, flag, resid);
   CG_LAUNCH_CHECK("cg_mlp_decode launch");
@@ -704,6 +714,7 @@ extern "C" int cg_mlp_decode(const float* Zh, int64_t ng, int32_t ns, const doub
 }
 
 extern "C" int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t* count, int64_t* idx, void* stream) {
+  if (ng < 0 || (ng > 0 && (!flag || !count || !idx))) return cg_internal_fail(CG_EINVAL, "cg_mlp_compact: bad argument");
   if (ng == 0) return CG_OK;
   compact_k<<<nblk(ng, 256), 256, 0, (hipStream_t)stream>>>(flag, ng, count, idx);
   CG_LAUNCH_CHECK("cg_mlp_compact launch");
@@ -713,6 +724,8 @@ extern "C" int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t* count, i
 extern "C" int cg_mlp_gather(const double* P, const double* T, const double* Y, int64_t ld,
                              const int64_t* idx, const int32_t* count, int64_t cap, int32_t ns,
                              double* Po, double* To, double* Yo, int64_t ldo, void* stream) {
+  if (cap < 0 || ns <= 0 || ldo < cap || (cap > 0 && (!P || !T || !Y || !idx || !count || !Po || !To || !Yo)))
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_gather: bad argument");
   if (cap == 0) return CG_OK;
   gather_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(P, T, Y, ld, idx, count, ns, Po, To, Yo, ldo);
   CG_LAUNCH_CHECK("cg_mlp_gather launch");
@@ -721,6 +734,8 @@ extern "C" int cg_mlp_gather(const double* P, const double* T, const double* Y, 
 
 extern "C" int cg_mlp_scatter(const double* Wi, int64_t ldi, const int64_t* idx, const int32_t* count,
                               int64_t cap, int32_t ns, double* W, int64_t ld, void* stream) {
+  if (cap < 0 || ns <= 0 || ldi < cap || (cap > 0 && (!Wi || !idx || !count || !W)))
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_scatter: bad argument");
   if (cap == 0) return CG_OK;
   scatter_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(Wi, ldi, idx, count, ns, W, ld);
   CG_LAUNCH_CHECK("cg_mlp_scatter launch");

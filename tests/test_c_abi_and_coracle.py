@@ -53,3 +53,36 @@ def test_c_oracle_matches_numpy_oracle(name):
     w = COracle(name).getrates(P, T, Y)
     st = parity_stats(m, w, ref, qf, qb)
     assert st["max_gross"] <= 1e-13, st
+
+
+def test_surrogate_entry_points_reject_bad_arguments_without_gpu():
+    """Negative sizes and missing buffers are refused before any launch."""
+    from chemgen_amd import surrogate
+    from chemgen_amd.runtime import lib
+    surrogate._bind()
+    L = lib()
+    x = 16                                   # a non-null stand-in: never dereferenced on these paths
+    cases = [
+        ("cg_mlp_targets", (x, 8, -1, 4, x, x, None)),
+        ("cg_mlp_targets", (x, 4, 8, 4, x, x, None)),           # leading dimension below ng
+        ("cg_mlp_targets", (None, 8, 8, 4, x, x, None)),
+        ("cg_mlp_mse_grad", (x, x, -1, 1.0, x, None, None)),
+        ("cg_mlp_silu_back", (x, x, -5, x, None)),
+        ("cg_mlp_colsum", (x, -1, 4, x, None)),
+        ("cg_mlp_colsum", (None, 4, 4, x, None)),
+        ("cg_mlp_adam", (x, x, x, x, -1, 1e-3, 0.9, 0.999, 1e-8, 1, None, None)),
+        ("cg_mlp_adam", (x, x, x, x, 8, 1e-3, 0.9, 0.999, 1e-8, 0, None, None)),
+        ("cg_mlp_to_bf16", (x, -1, x, None)),
+        ("cg_mlp_decode", (x, -1, 4, x, x, 2, 1e-2, x, 8, x, x, None)),
+        ("cg_mlp_decode", (x, 8, 4, x, x, 2, 1e-2, x, 8, None, x, None)),
+        ("cg_mlp_compact", (x, -1, x, x, None)),
+        ("cg_mlp_gather", (x, x, x, 8, x, x, -1, 4, x, x, x, 8, None)),
+        ("cg_mlp_gather", (x, x, x, 8, x, x, 8, 4, x, x, x, 4, None)),
+        ("cg_mlp_scatter", (x, 8, x, x, -1, 4, x, 8, None)),
+        ("cg_mlp_scatter", (x, 8, None, x, 8, 4, x, 8, None)),
+    ]
+    for name, args in cases:
+        assert getattr(L, name)(*args) == -1, (name, args)
+        assert b"bad argument" in L.cg_last_error(), name
+    assert L.cg_mlp_decode(x, 8, 4, x, x, 9, 1e-2, x, 8, x, x, None) != 0
+    assert b"at most 8" in L.cg_last_error()
