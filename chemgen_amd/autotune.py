@@ -60,7 +60,9 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
                 out += [Variant(64, wpe, "lean", "gh", "horner", False, 4, 1, 0, False, True),
                         Variant(64, wpe, "lean", "gh", "horner", False, 4, 1, 0, True, True)]
     if table_supported(mech):
-        for npt in (16, 32, 64):
+        # tiles shorter than 16 points only where a 16-point tile does not fit
+        short = (1, 2, 4, 8) if table_lds_bytes(ns, 16, 4) > LDS_BYTES else ()
+        for npt in short + (16, 32, 64):
             for waves, wpe in ((4, 1), (8, 2), (16, 4)):
                 if table_lds_bytes(ns, npt, waves) <= LDS_BYTES:
                     out.append(Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt))
@@ -68,8 +70,8 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
 
 
 def table_supported(mech: Mechanism) -> bool:
-    """Whether the table kernel's reaction format holds this mechanism (no PLOG, integer
-    stoichiometry up to 3, ...)."""
+    """Whether the table kernel's reaction format holds this mechanism (reaction types up to
+    PLOG, at most 15 collision efficiencies per reaction, ...)."""
     from .codegen import table_records
     try:
         table_records(mech)
@@ -80,10 +82,15 @@ def table_supported(mech: Mechanism) -> bool:
 
 def default_variant(mech: Mechanism) -> Variant:
     """Shape used before any tuning: lane per point for small mechanisms, the table kernel
-    (16-point tiles, 8 wavefronts) for large ones."""
+    for large ones (8 wavefronts, the longest tile of at most 16 points whose species state
+    fits the LDS; 4 wavefronts when 8 do not fit)."""
     if mech.n_species <= STRAIGHT_LINE_MAX_SPECIES:
         return Variant()
-    return Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 16)
+    for npt in (16, 8, 4, 2, 1):
+        for waves, wpe in ((8, 2), (4, 1)):
+            if table_lds_bytes(mech.n_species, npt, waves) <= LDS_BYTES:
+                return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt)
+    raise ValueError(f"{mech.n_species} species: the table kernel's one-point tile does not fit the LDS")
 
 
 def _load_db() -> Dict:

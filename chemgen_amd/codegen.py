@@ -130,7 +130,11 @@ def _lk(off: int, beta: float, ea: float) -> str:
 
 
 def _arr_vals(A: float, beta: float, Ea: float) -> Tuple[float, float, float]:
-    return (math.log2(r15(A)), float(beta), r15(Ea) / RC * LOG2E)
+    """(log2 A, beta, Ea / (Rc ln 2)); A = 0 (a placeholder rate, kf = 0) is stored as
+    log2 A = -1100, which exp2 takes to 0."""
+    if A < 0.0:
+        raise ValueError(f"Arrhenius prefactor {A} < 0 outside a PLOG sum")
+    return (math.log2(r15(A)) if A > 0.0 else -1100.0, float(beta), r15(Ea) / RC * LOG2E)
 
 
 def structure_key(mech: Mechanism) -> str:
@@ -854,7 +858,19 @@ def _emit_split(mech: Mechanism, variant: Variant):
     return src, coef, meta
 
 
-_TABLE_KIND = {"standard": 0, "third_body": 1, "lindemann": 2, "troe": 3, "sri": 5}
+_TABLE_KIND = {"standard": 0, "third_body": 1, "lindemann": 2, "troe": 3, "sri": 5, "plog": 7}
+TABLE_LONG = 1 << 12      # class bit of a reaction written as an extended record
+
+
+def _table_long(r: Reaction) -> bool:
+    """Whether a reaction needs the extended record: a side with more than three species
+    slots, or a non-integer stoichiometric coefficient."""
+    for side in (r.reactants, r.products):
+        if any(not float(nu).is_integer() for nu in side.values()):
+            return True
+        if sum(int(nu) for nu in side.values()) > 3:
+            return True
+    return False
 
 
 def _table_slots(r: Reaction, idx: Dict[str, int]) -> Tuple[List[int], List[int]]:
@@ -863,10 +879,10 @@ def _table_slots(r: Reaction, idx: Dict[str, int]) -> Tuple[List[int], List[int]
         sl: List[int] = []
         for sp, nu in side.items():
             if not float(nu).is_integer():
-                raise ValueError(f"table kernel: stoichiometric coefficient {nu}")
+                raise ValueError(f"table kernel: stoichiometric coefficient {nu} needs an extended record")
             sl += [idx[sp]] * int(nu)
         if not 1 <= len(sl) <= 3:
-            raise ValueError("table kernel: a reaction side needs 1 to 3 species slots")
+            raise ValueError("table kernel: a side of more than 3 species slots needs an extended record")
         out.append(sl)
     return out[0], out[1]
 
@@ -883,11 +899,14 @@ def _table_kind(r: Reaction) -> int:
 def table_class(mech: Mechanism, j: int) -> Tuple[int, int, int, int, int]:
     """The shape the table kernel compiles a loop for: (family, reversible, reactant slots,
     product slots, dnu if reversible else 0); family 0 elementary, 1 third-body,
-    2 Lindemann, 3 Troe (with or without T2), 5 SRI (with or without e)."""
+    2 Lindemann, 3 Troe (with or without T2), 5 SRI (with or without e), 7 PLOG.  A reaction
+    that needs an extended record has slot counts 0 (it never gets a loop of its own)."""
     r = mech.reactions[j]
+    fam = _TABLE_KIND[r.rtype]
+    if _table_long(r):
+        return (fam, int(r.reversible), 0, 0, 0)
     idx = {s.name: i for i, s in enumerate(mech.species)}
     rs, ps = _table_slots(r, idx)
-    fam = _TABLE_KIND[r.rtype]
     return (fam, int(r.reversible), len(rs), len(ps), (len(ps) - len(rs)) if r.reversible else 0)
 
 
@@ -895,39 +914,61 @@ def table_order(mech: Mechanism, min_class: int = 0):
     """Processing order of the table kernel and its class segments.  Reactions of a class
     (``table_class``) with at least ``min_class`` members are contiguous and get a loop of
     their own, compiled for that shape; the rest follow in one generic segment, sorted by
-    (kind, reversibility, slot counts, efficiency count) so that the lane groups of one
-    wavefront mostly take the same branches.  Returns (order, [(start, end, class)...],
-    start of the generic segment)."""
+    (extended record, kind, reversibility, slot counts, efficiency count) so that the lane
+    groups of one wavefront mostly take the same branches.  Returns (order, [(start, end,
+    class)...], start of the generic segment)."""
     by: Dict[Tuple, List[int]] = {}
     for j in range(mech.n_reactions):
         by.setdefault(table_class(mech, j), []).append(j)
     order: List[int] = []
     segs = []
+    compiled = lambda key: key[2] > 0 and len(by[key]) >= max(1, min_class)
     for key in sorted(by):
-        if len(by[key]) >= max(1, min_class):
+        if compiled(key):
             segs.append((len(order), len(order) + len(by[key]), key))
             order.extend(by[key])
-    rest = sorted((j for key in by if len(by[key]) < max(1, min_class) for j in by[key]),
-                  key=lambda j: (_table_kind(mech.reactions[j]), mech.reactions[j].reversible,
-                                 table_class(mech, j)[2:4], len(mech.reactions[j].third_body), j))
+    rest = sorted((j for key in by if not compiled(key) for j in by[key]),
+                  key=lambda j: (_table_long(mech.reactions[j]), _table_kind(mech.reactions[j]),
+                                 mech.reactions[j].reversible, table_class(mech, j)[2:4],
+                                 len(mech.reactions[j].third_body), j))
     gen_start = len(order)
     order.extend(rest)
     return order, segs, gen_start
 
 
-def table_records(mech: Mechanism, npt: int = 16, min_class: int = 0):
-    """(reaction table rows [2 nr][4], efficiency species, order, class segments, start of the
-    generic segment) for the table kernel with ``npt``-point tiles.  Each side of a reaction is
+@dataclass
+class TableLayout:
+    """What the table kernel reads besides the gpu_coef blob (see ``table_layout``)."""
+    rows: List[Tuple[int, int, int, int]]
+    effsp: List[int]
+    order: List[int]
+    segs: List[Tuple[int, int, Tuple]]
+    gen_start: int
+    plog: List[Tuple[int, int]]   # per PLOG pressure point: (blob offset of the point, rates)
+    xsl: List[int]                # extended records: [n reactant, n product entries, rows...]
+    xnu: List[float]              # parallel to xsl: [dnu, 0, nu of each entry...]
+
+
+def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLayout:
+    """The table kernel's reaction table for ``npt``-point tiles.  Each side of a reaction is
     written as up to three species *slots* (a species with coefficient nu fills nu slots),
     absent slots point at an extra LDS row that holds EG = h = 1; slots are stored as LDS row
     offsets (species index x npt).
       row 0: x = blob offset of the reaction's record, y = r0 | r1 << 16, z = r2 | p0 << 16,
              w = p1 | p2 << 16
       row 1: x = kind | reversible << 3 | reactant slots << 4 | product slots << 6
-                 | efficiencies << 8 | (dnu + 8) << 24, y = offset of the collision-efficiency
-             species in the efficiency list, z = blob offset of their (eff - 1) values, w = 0
-    Raises ValueError for what the format does not hold (more than 3 slots per side,
-    non-integer coefficients, more than 15 efficiencies, PLOG)."""
+                 | efficiencies << 8 | extended << 12 | (dnu + 8) << 24,
+             y = offset of the collision-efficiency species in the efficiency list (PLOG: of
+             its pressure points in the PLOG list), z = blob offset of their (eff - 1) values
+             (PLOG: the number of pressure points), w = offset of an extended record in
+             xsl / xnu (0 otherwise)
+    kind: 0 elementary, 1 third-body, 2 Lindemann, 3 Troe, 4 Troe with T2, 5 SRI, 6 SRI with
+    e, 7 PLOG.  A reaction with a side of more than three slots or a non-integer coefficient
+    is an extended record: its row-0 slots are unused (the padding row), and xsl[w:] holds the
+    number of reactant and of product entries and then each entry's LDS row offset, xnu[w:]
+    its dnu (sum of product minus reactant coefficients), 0 and each entry's coefficient.
+    Raises ValueError for what the format does not hold (more than 15 efficiencies, an LDS
+    row offset beyond 16 bits)."""
     coef, off_smh, roffs = build_coef(mech)
     idx = {s.name: i for i, s in enumerate(mech.species)}
     pad = mech.n_species
@@ -940,23 +981,53 @@ def table_records(mech: Mechanism, npt: int = 16, min_class: int = 0):
             raise ValueError("table kernel: more than 15 collision efficiencies")
     rows: List[Tuple[int, int, int, int]] = []
     effsp: List[int] = []
+    plog: List[Tuple[int, int]] = []
+    xsl: List[int] = [0, 0]           # offset 0: no extended record
+    xnu: List[float] = [0.0, 0.0]
     order, segs, gen_start = table_order(mech, min_class)
     for j in order:
         r = mech.reactions[j]
         kind = _table_kind(r)
-        rsl, psl = _table_slots(r, idx)
-        nrs, nps = len(rsl), len(psl)
-        dnu = nps - nrs
+        lng = _table_long(r)
+        if lng:
+            nrs = nps = dnu = 0
+            rr = pp = [pad * npt] * 3
+            xo = len(xsl)
+            xsl += [len(r.reactants), len(r.products)]
+            xsl += [idx[sp] * npt for sp in r.reactants] + [idx[sp] * npt for sp in r.products]
+            xnu += [float(r.coeff_sum), 0.0]
+            xnu += [float(nu) for nu in r.reactants.values()] + [float(nu) for nu in r.products.values()]
+        else:
+            rsl, psl = _table_slots(r, idx)
+            nrs, nps = len(rsl), len(psl)
+            dnu = nps - nrs
+            rr = [k * npt for k in rsl] + [pad * npt] * (3 - nrs)
+            pp = [k * npt for k in psl] + [pad * npt] * (3 - nps)
+            xo = 0
         x = kind | (int(r.reversible) << 3) | (nrs << 4) | (nps << 6) | (len(r.third_body) << 8) \
-            | ((dnu + 8) << 24)
-        rr = [k * npt for k in rsl] + [pad * npt] * (3 - nrs)
-        pp = [k * npt for k in psl] + [pad * npt] * (3 - nps)
-        nx = {"standard": 0, "third_body": 0, "lindemann": 3, "troe": 3 + (5 if len(r.troe or ()) == 4 else 4),
-              "sri": 3 + 5}[r.rtype]
+            | (TABLE_LONG if lng else 0) | ((dnu + 8) << 24)
         rows.append((roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16), pp[1] | (pp[2] << 16)))
-        rows.append((x, len(effsp), roffs[j] + 3 + nx, 0))
-        effsp.extend(idx[sp] for sp in r.third_body)
-    return rows, effsp, order, segs, gen_start
+        if r.rtype == "plog":
+            pts = sorted(set(c[0] for c in r.plog))
+            y, o = len(plog), roffs[j]
+            for pt in pts:
+                n = len([c for c in r.plog if c[0] == pt])
+                plog.append((o, n))
+                o += 3 + 4 * n
+            rows.append((x, y, len(pts), xo))
+        else:
+            nx = {"standard": 0, "third_body": 0, "lindemann": 3,
+                  "troe": 3 + (5 if len(r.troe or ()) == 4 else 4), "sri": 3 + 5}[r.rtype]
+            rows.append((x, len(effsp), roffs[j] + 3 + nx, xo))
+            effsp.extend(idx[sp] for sp in r.third_body)
+    return TableLayout(rows, effsp, order, segs, gen_start, plog, xsl, xnu)
+
+
+def table_records(mech: Mechanism, npt: int = 16, min_class: int = 0):
+    """(reaction table rows [2 nr][4], efficiency species, order, class segments, start of the
+    generic segment) of ``table_layout``."""
+    t = table_layout(mech, npt, min_class)
+    return t.rows, t.effsp, t.order, t.segs, t.gen_start
 
 
 def _emit_table(mech: Mechanism, variant: Variant):
@@ -966,9 +1037,13 @@ def _emit_table(mech: Mechanism, variant: Variant):
     if 64 % npt:
         raise ValueError("table kernel: points per tile must divide 64")
     slots = W * 64 // npt
+    lds = 8 * (3 * (mech.n_species + 1) * npt + slots * npt)
+    if lds > LDS_PER_CU:
+        raise ValueError(f"table kernel: {npt}-point tiles of {mech.n_species} species need {lds} B of LDS")
     # a class gets a compiled loop when it fills at least half the reaction slots of a tile
     # (JetSurf-sized: 2.6% faster than a full round of slots, every class compiled 5% slower)
-    rows, effsp, order, segs, gen_start = table_records(mech, npt, min_class=max(1, slots // 2))
+    lay = table_layout(mech, npt, min_class=max(1, slots // 2))
+    rows, effsp, segs, gen_start = lay.rows, lay.effsp, lay.segs, lay.gen_start
     classes = []
     for (a, b, (fam, rev, nrs, nps, dnu)) in segs:
         pf = None
@@ -989,11 +1064,15 @@ def _emit_table(mech: Mechanism, variant: Variant):
         log10_2=dlit(LOG10_2), log2_10=dlit(LOG2_10), exp_impl=variant.exp_impl, poly=variant.poly,
         rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
         classes=classes, gen_start=gen_start,
+        has_plog=bool(lay.plog), plog=", ".join(f"{{{o}, {n}}}" for o, n in lay.plog), n_plog=len(lay.plog),
+        has_long=len(lay.xsl) > 2, xsl=", ".join(str(v) for v in lay.xsl), n_xsl=len(lay.xsl),
+        xnu=", ".join(dlit(v) for v in lay.xnu),
         tmid=", ".join(dlit(s.thermo.t_mid) for s in mech.species),
     )
     meta = dict(ns=mech.n_species, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads,
                 ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, structure_hash=h, slots=slots,
-                compiled_classes=len(classes), generic_reactions=mech.n_reactions - gen_start)
+                compiled_classes=len(classes), generic_reactions=mech.n_reactions - gen_start,
+                lds_bytes=8 * (3 * (mech.n_species + 1) * npt + slots * npt))
     return src, coef, meta
 
 

@@ -17,7 +17,17 @@ def load_bundled(name: str) -> Mechanism:
     return parse_chemkin(os.path.join(DATA_DIR, name + ".inp"), name=name)
 
 
-TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348", "h2_burke")
+TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348", "h2_burke", "fuller2021")
+
+# table-kernel shapes compiled for small mechanisms whose tuned kernel is another family, so
+# that the table format's PLOG and extended records are exercised: 16-point tiles of 4
+# wavefronts (compiled class loops and the generic loop) and 4-point tiles of 8
+TABLE_CHECKED = {
+    "synth_plog53": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 16),
+                     Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
+    "synth_ext53": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 16),
+                    Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
+}
 
 
 def default_variants(mech: Mechanism) -> List[Variant]:
@@ -31,4 +41,5 @@ def default_variants(mech: Mechanism) -> List[Variant]:
     live = Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True)
     if mech.n_species <= STRAIGHT_LINE_MAX_SPECIES and live not in out:
         out.append(live)   # the live-range-ordered lane kernel, for its parity tests
+    out += [v for v in TABLE_CHECKED.get(mech.name, ()) if v not in out]
     return out

@@ -177,6 +177,42 @@ def synthetic_mechanism(base_species: Sequence[Species], elements: Sequence[str]
     return Mechanism(name, els, species, reactions)
 
 
+def with_extended_reactions(mech: Mechanism, n: int, seed: int, name: str) -> Mechanism:
+    """``mech`` plus ``n`` element-balanced reactions of the shapes real mechanisms hold beyond
+    three species per side and integer coefficients: the sum of two of its reactions (sides
+    of four or more species; repeated species merge into one coefficient) or half of one
+    (coefficients 0.5), cycling over the kinds of the reaction taken first (elementary,
+    third-body, falloff, PLOG) so that each kind appears with such sides."""
+    import copy
+    rng = np.random.default_rng(seed)
+    base = list(mech.reactions)
+    by_kind: Dict[str, List[Reaction]] = {}
+    for r in base:
+        by_kind.setdefault(r.rtype, []).append(r)
+    kinds = [k for k in ("standard", "third_body", "troe", "plog", "lindemann", "sri") if k in by_kind]
+    simple = by_kind["standard"]
+
+    def add(a: Dict[str, float], b: Dict[str, float], f: float = 1.0) -> Dict[str, float]:
+        out = dict(a)
+        for s2, nu in b.items():
+            out[s2] = out.get(s2, 0.0) + f * nu
+        return out
+
+    extra: List[Reaction] = []
+    for i in range(n):
+        r = copy.deepcopy(by_kind[kinds[i % len(kinds)]][int(rng.integers(len(by_kind[kinds[i % len(kinds)]])))])
+        if i % 2 == 0:
+            o = simple[int(rng.integers(len(simple)))]
+            r.reactants, r.products = add(r.reactants, o.reactants), add(r.products, o.products)
+        else:
+            r.reactants = {k: 0.5 * v for k, v in r.reactants.items()}
+            r.products = {k: 0.5 * v for k, v in r.products.items()}
+        r.reversible = bool(i % 3 != 2)
+        r.duplicate = True
+        extra.append(r)
+    return Mechanism(name, list(mech.elements), list(mech.species), base + extra)
+
+
 def write_and_parse(mech: Mechanism, path: str) -> Mechanism:
     write_chemkin(mech, path)
     return parse_chemkin(path, name=mech.name)

@@ -37,6 +37,8 @@ def _run(torch, mech, P, T, Y, dm=None):
     ("synth_nhept88", 1024, (600.0, 2800.0)),
     ("synth_jetsurf348", 512, (600.0, 2800.0)),
     ("synth_plog53", 2048, (300.0, 2500.0)),
+    ("synth_ext53", 2048, (300.0, 2500.0)),
+    ("fuller2021", 256, (600.0, 2800.0)),
 ])
 def test_parity_vs_oracle(torch_cuda, name, ng, trange):
     m = load_bundled(name)
@@ -60,17 +62,21 @@ def test_ragged_tail_sizes(torch_cuda, ng):
     assert parity_stats(m, w, ref, qf, qb)["max_gross"] <= TOL_GROSS
 
 
-@pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348"])
+@pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53", "synth_ext53",
+                                  "fuller2021"])
 def test_table_kernels_vs_oracle(torch_cuda, name):
-    """Every table-driven candidate (all tile lengths and wavefront counts that fit the LDS)
-    against the oracle; gri30 brings Troe with and without T2 and Lindemann reactions, the
-    JetSurf-sized mechanism SRI; 300 points leave a partial tile for every tile length."""
-    from chemgen_amd.autotune import candidate_variants
+    """Every table-driven kernel built for the mechanism (all tile lengths and wavefront counts
+    that fit the LDS) against the oracle; gri30 brings Troe with and without T2 and Lindemann
+    reactions, the JetSurf-sized mechanism SRI, synth_plog53 PLOG reactions (in compiled class
+    loops and the generic loop), synth_ext53 extended records (sides of 4-5 species slots,
+    coefficients 0.5, with PLOG, third-body and falloff rates), Fuller 2021 (835 species) all of
+    them in 1- to 8-point tiles; 300 points leave a partial tile for every tile length."""
+    from chemgen_amd.kernels import default_variants
     from chemgen_amd.runtime import DeviceMechanism
     m = load_bundled(name)
     P, T, Y = make_states(m.n_species, 300, seed=4, T_range=(300.0, 2500.0))
     ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
-    tables = [v for v in candidate_variants(m) if v.table]
+    tables = [v for v in default_variants(m) if v.table]
     assert tables
     for v in tables:
         print(f"[table] {name} {v.tag}", flush=True)
@@ -144,15 +150,20 @@ def _edge_states(m):
 
 
 @pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_nhept88", "synth_jetsurf348",
-                                  "synth_plog53"])
+                                  "synth_plog53", "synth_ext53", "fuller2021"])
 def test_edge_states_vs_oracle(torch_cuda, name):
-    """The tuned kernel of each mechanism on the crafted states of _edge_states."""
+    """The tuned kernel of each mechanism on the crafted states of _edge_states.  Where the
+    oracle itself overflows (Fuller 2021 at 3000 K and above: exp(smh) of species whose NASA-7
+    fits end below that) the point is left out."""
     m = load_bundled(name)
     P, T, Y = _edge_states(m)
     print(f"[edge] {name} ng={P.shape[0]}", flush=True)
     w = _run(torch_cuda, m, P, T, Y)
-    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
-    st = parity_stats(m, w, ref, qf, qb)
+    with np.errstate(over="ignore", invalid="ignore"):
+        ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    keep = np.isfinite(ref).all(axis=0) & np.isfinite(qf).all(axis=0) & np.isfinite(qb).all(axis=0)
+    assert keep.mean() > 0.95 and (keep.all() or name == "fuller2021")
+    st = parity_stats(m, w[:, keep], ref[:, keep], qf[:, keep], qb[:, keep])
     assert st["finite"] and st["max_gross"] <= TOL_GROSS, st
 
 

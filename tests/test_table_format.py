@@ -5,50 +5,98 @@ from collections import Counter
 import pytest
 
 from chemgen_amd.autotune import table_supported
-from chemgen_amd.codegen import build_coef, table_class, table_records
+from chemgen_amd.codegen import TABLE_LONG, build_coef, table_class, table_layout, table_records
 from chemgen_amd.kernels import load_bundled
 
 
-@pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_jetsurf348"])
-@pytest.mark.parametrize("npt,min_class", [(16, 64), (64, 0), (16, 1000)])
+@pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53",
+                                  "synth_ext53", "fuller2021"])
+@pytest.mark.parametrize("npt,min_class", [(16, 64), (64, 0), (16, 1000), (2, 8)])
 def test_table_decodes_to_mechanism(name, npt, min_class):
     m = load_bundled(name)
-    rows, effsp, order, segs, gen_start = table_records(m, npt, min_class)
+    if (m.n_species + 1) * npt >= 1 << 16:
+        pytest.skip("LDS row offsets beyond 16 bits")
+    lay = table_layout(m, npt, min_class)
+    rows, effsp, order, segs, gen_start = lay.rows, lay.effsp, lay.order, lay.segs, lay.gen_start
+    assert table_records(m, npt, min_class) == (rows, effsp, order, segs, gen_start)
     _, _, roffs = build_coef(m)
     pad = m.n_species * npt
     idx = {s.name: i for i, s in enumerate(m.species)}
     assert sorted(order) == list(range(m.n_reactions))
-    kinds = {"standard": {0}, "third_body": {1}, "lindemann": {2}, "troe": {3, 4}, "sri": {5, 6}}
+    kinds = {"standard": {0}, "third_body": {1}, "lindemann": {2}, "troe": {3, 4}, "sri": {5, 6}, "plog": {7}}
+    n_long = 0
     for n, j in enumerate(order):
         r = m.reactions[j]
-        (y, z, w, x2), (x, e0, e1, _) = rows[2 * n], rows[2 * n + 1]
+        (y, z, w, x2), (x, e0, e1, xo) = rows[2 * n], rows[2 * n + 1]
         assert (x & 7) in kinds[r.rtype]
         assert ((x >> 3) & 1) == int(r.reversible)
         nrs, nps, ne, dnu = (x >> 4) & 3, (x >> 6) & 3, (x >> 8) & 15, ((x >> 24) & 15) - 8
+        lng = bool(x & TABLE_LONG)
         assert y == roffs[j]
         slots_r = [z & 0xFFFF, z >> 16, w & 0xFFFF]
         slots_p = [w >> 16, x2 & 0xFFFF, x2 >> 16]
-        for slots, count, side in ((slots_r, nrs, r.reactants), (slots_p, nps, r.products)):
-            assert all(s % npt == 0 for s in slots)
-            assert slots[count:] == [pad] * (3 - count)
-            assert Counter(s // npt for s in slots[:count]) == Counter({idx[k]: int(v) for k, v in side.items()})
-        assert dnu == int(r.coeff_sum)
+        if lng:
+            # extended record: entries with their coefficients, dnu in xnu
+            n_long += 1
+            assert nrs == nps == 0 and slots_r + slots_p == [pad] * 6 and xo > 0
+            a, b = lay.xsl[xo], lay.xsl[xo + 1]
+            ent = list(zip(lay.xsl[xo + 2:xo + 2 + a + b], lay.xnu[xo + 2:xo + 2 + a + b]))
+            assert all(s % npt == 0 for s, _ in ent)
+            assert {s // npt: nu for s, nu in ent[:a]} == {idx[k]: v for k, v in r.reactants.items()}
+            assert {s // npt: nu for s, nu in ent[a:]} == {idx[k]: v for k, v in r.products.items()}
+            assert lay.xnu[xo] == r.coeff_sum
+            assert any(not float(v).is_integer() for v in list(r.reactants.values()) + list(r.products.values())) \
+                or max(sum(r.reactants.values()), sum(r.products.values())) > 3
+        else:
+            assert xo == 0
+            for slots, count, side in ((slots_r, nrs, r.reactants), (slots_p, nps, r.products)):
+                assert all(s % npt == 0 for s in slots)
+                assert slots[count:] == [pad] * (3 - count)
+                assert Counter(s // npt for s in slots[:count]) == Counter({idx[k]: int(v) for k, v in side.items()})
+            assert dnu == int(r.coeff_sum)
         assert ne == len(r.third_body)
-        assert effsp[e0:e0 + ne] == [idx[k] for k in r.third_body]
-    # class segments: contiguous, one shape each, at least min_class long, then the generic rest
+        if r.rtype == "plog":
+            # pressure points: blob offsets walk the record, rates per point as in the mechanism
+            pts = sorted(set(c[0] for c in r.plog))
+            assert e1 == len(pts)
+            o = roffs[j]
+            for q, p in enumerate(pts):
+                nrates = len([c for c in r.plog if c[0] == p])
+                assert lay.plog[e0 + q] == (o, nrates)
+                o += 3 + 4 * nrates
+        else:
+            assert effsp[e0:e0 + ne] == [idx[k] for k in r.third_body]
+    assert n_long == sum(1 for r in m.reactions
+                         if any(not float(v).is_integer() for v in list(r.reactants.values()) + list(r.products.values()))
+                         or max(sum(r.reactants.values()), sum(r.products.values())) > 3)
+    if name in ("synth_ext53", "fuller2021"):
+        assert n_long >= 12
+    # class segments: contiguous, one shape each, at least min_class long, then the generic rest;
+    # extended records never get a loop of their own
     pos = 0
     for a, b, key in segs:
-        assert a == pos and b - a >= max(1, min_class)
+        assert a == pos and b - a >= max(1, min_class) and key[2] > 0
         assert all(table_class(m, order[n]) == key for n in range(a, b))
         pos = b
     assert gen_start == pos
     seg_keys = {key for _, _, key in segs}
     assert all(table_class(m, order[n]) not in seg_keys for n in range(gen_start, m.n_reactions))
+    longs = [bool(rows[2 * n + 1][0] & TABLE_LONG) for n in range(gen_start, m.n_reactions)]
+    assert longs == sorted(longs)     # extended records last in the generic segment
 
 
-def test_plog_mechanism_is_not_tabled():
-    assert not table_supported(load_bundled("synth_plog53"))
-    assert table_supported(load_bundled("synth_jetsurf348"))
+def test_every_bundled_mechanism_is_tabled():
+    for name in ("synth_plog53", "synth_ext53", "synth_jetsurf348", "fuller2021"):
+        assert table_supported(load_bundled(name)), name
+
+
+def test_default_variant_fits_the_lds():
+    """Large mechanisms get the longest tile of at most 16 points whose state fits 160 KB."""
+    from chemgen_amd.autotune import LDS_BYTES, default_variant, table_lds_bytes
+    for name, npt in (("synth_jetsurf348", 16), ("fuller2021", 8)):
+        v = default_variant(load_bundled(name))
+        assert v.table == npt, (name, v)
+        assert table_lds_bytes(load_bundled(name).n_species, v.table, v.split) <= LDS_BYTES
 
 
 @pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_plog53"])
