@@ -145,9 +145,11 @@ def bench_batched(steps, warmup):
             "parity": {"sample_points": int(len(pick)), **par}}
 
 
-def bench_large(steps, warmup, ng=1 << 19):
-    """Plain FP64 getrates on the JetSurf-sized mechanism (348 species, 2163 reactions) over
-    2^19 points, T U(600, 2800) K, with the tuned (table-driven) kernel."""
+def bench_large(steps, warmup, ng=1 << 19, name="synth_jetsurf348", nchk=256):
+    """Plain FP64 getrates on a large mechanism, T U(600, 2800) K, with the tuned
+    (table-driven) kernel: the JetSurf-sized synthetic mechanism (348 species, 2163 reactions)
+    over 2^19 points, or Fuller et al. 2021 (835 species, 4289 reactions with 523 PLOG) over
+    2^18."""
     import numpy as np
     import torch
     from chemgen_amd.autotune import best_variant
@@ -157,7 +159,6 @@ def bench_large(steps, warmup, ng=1 << 19):
     from chemgen_amd.synth import make_states
     from oracle.getrates_np import getrates as np_oracle
     from oracle.metrics import parity_stats
-    name = "synth_jetsurf348"
     mech = load_bundled(name)
     variant = best_variant(mech)
     P, T, Y = make_states(mech.n_species, ng, seed=0, T_range=(600.0, 2800.0))
@@ -174,7 +175,6 @@ def bench_large(steps, warmup, ng=1 << 19):
     e1.record(s)
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) / steps / 1e3
-    nchk = 256
     Wc = Wd[:, :nchk].cpu().numpy()
     ref, qf, qb = np_oracle(mech, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
     par = parity_stats(mech, Wc, ref, qf, qb)
@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
-    ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized workload")
+    ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized and Fuller 2021 workloads")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate workload")
     ap.add_argument("--no-h2", action="store_true", help="skip the H2 Burke mini-app workload")
     ap.add_argument("--no-qss", action="store_true", help="skip the QSS-species workload")
@@ -486,6 +486,9 @@ def main():
     if not args.no_large:
         torch.cuda.empty_cache()
         out["large"] = bench_large(max(3, args.steps // 4), min(args.warmup, 2))
+        torch.cuda.empty_cache()
+        out["large_real"] = bench_large(max(3, args.steps // 4), min(args.warmup, 2), ng=1 << 18,
+                                        name="fuller2021", nchk=128)
     if not args.no_surrogate:
         torch.cuda.empty_cache()
         out["surrogate"] = bench_surrogate(args.steps, args.warmup)

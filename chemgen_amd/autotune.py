@@ -28,8 +28,8 @@ STRAIGHT_LINE_MAX_SPECIES = 150  # above this the lane-per-point and split kerne
 
 def table_lds_bytes(ns: int, npt: int, waves: int) -> int:
     """LDS of one table-kernel tile: EG, h, wdot [ns + 1][npt] (one row of ones for absent
-    species slots) and the per-slot partial sums."""
-    return 8 * (3 * (ns + 1) * npt + waves * 64)
+    species slots) and the per-wavefront partial sums."""
+    return 8 * (3 * (ns + 1) * npt + waves * npt)
 
 
 def candidate_variants(mech: Mechanism) -> List[Variant]:

@@ -1037,7 +1037,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
     if 64 % npt:
         raise ValueError("table kernel: points per tile must divide 64")
     slots = W * 64 // npt
-    lds = 8 * (3 * (mech.n_species + 1) * npt + slots * npt)
+    lds = 8 * (3 * (mech.n_species + 1) * npt + W * npt)
     if lds > LDS_PER_CU:
         raise ValueError(f"table kernel: {npt}-point tiles of {mech.n_species} species need {lds} B of LDS")
     # a class gets a compiled loop when it fills at least half the reaction slots of a tile
@@ -1072,7 +1072,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
     meta = dict(ns=mech.n_species, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads,
                 ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, structure_hash=h, slots=slots,
                 compiled_classes=len(classes), generic_reactions=mech.n_reactions - gen_start,
-                lds_bytes=8 * (3 * (mech.n_species + 1) * npt + slots * npt))
+                lds_bytes=lds)
     return src, coef, meta
 
 

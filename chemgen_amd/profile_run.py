@@ -5,6 +5,15 @@ the ragged-batch launch.
     python -m chemgen_amd.profile_run getrates --mech synth_gri53 --ng 1048576 --reps 5
     python -m chemgen_amd.profile_run infer --mech synth_gri53 --ng 1048576 --reps 5
     python -m chemgen_amd.profile_run batched --mech synth_nhept88 --reps 5
+    python -m chemgen_amd.profile_run gemm --mech fuller2021 --ng 65536 --reps 5
+    python -m chemgen_amd.profile_run gemm --shape 3761 16532 --ng 16384 --reps 5
+
+``gemm`` times the GEMM formulation of a mechanism's stoichiometric products -- the logs of
+the forward and reverse rate-of-progress products, [2 Nr x Ns] x log C [Ns x ng], and the
+production rates, nu [Ns x Nr] x q [Nr x ng] -- as dense FP64 GEMMs (torch.matmul, rocBLAS
+on the FP64 matrix cores), and, for a bundled mechanism, the tuned getrates kernel at the
+same ng; it prints one JSON line with both times per point.  ``--shape NS NR`` takes a
+mechanism of that size with two reactants and two products per reaction.
 
 States are drawn on the GPU (T U(800, 2800) K unless --tlo/--thi, P 1-40 atm, Y Dirichlet(0.5)).
 """
@@ -32,7 +41,7 @@ def main(argv=None):
     from .runtime import DeviceMechanism
     from .surrogate import Surrogate, random_states
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["getrates", "infer", "batched"])
+    ap.add_argument("what", choices=["getrates", "infer", "batched", "gemm"])
     ap.add_argument("--mech", default="synth_gri53")
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--reps", type=int, default=5)
@@ -41,7 +50,11 @@ def main(argv=None):
     ap.add_argument("--weights", default=None, help="surrogate weights (.npz); default: the bundled ones")
     ap.add_argument("--warm", type=float, default=0.0,
                     help="seconds of untimed launches before the --reps ones (clocks settle under load)")
+    ap.add_argument("--shape", type=int, nargs=2, default=None, metavar=("NS", "NR"),
+                    help="gemm: mechanism size without a mechanism (two reactants, two products each)")
     a = ap.parse_args(argv)
+    if a.what == "gemm":
+        return _gemm(a)
     m = load_bundled(a.mech)
     if a.what == "batched":
         from .synth import ragged_patches
@@ -78,6 +91,64 @@ def main(argv=None):
         tag = f"MLP {dims}"
     torch.cuda.synchronize()
     print(f"done {a.what} {a.mech} {tag}", flush=True)
+
+
+def _gemm(a):
+    """The stoichiometric GEMMs of the GEMM formulation against the getrates kernel."""
+    import json
+    import numpy as np
+    import torch
+    from .autotune import _time_ms, best_variant
+    from .kernels import load_bundled
+    from .runtime import DeviceMechanism
+    from .surrogate import random_states
+    dev = torch.device("cuda:0")
+    if a.shape:
+        ns, nr = a.shape
+        rng = np.random.default_rng(0)
+        nu_r = np.zeros((nr, ns))
+        nu_p = np.zeros((nr, ns))
+        for j in range(nr):
+            nu_r[j, rng.choice(ns, 2, replace=False)] = 1.0
+            nu_p[j, rng.choice(ns, 2, replace=False)] = 1.0
+        name, m = f"{ns}x{nr}", None
+    else:
+        m = load_bundled(a.mech)
+        ns, nr, name = m.n_species, m.n_reactions, a.mech
+        idx = {s.name: i for i, s in enumerate(m.species)}
+        nu_r = np.zeros((nr, ns))
+        nu_p = np.zeros((nr, ns))
+        for j, r in enumerate(m.reactions):
+            for sp, nu in r.reactants.items():
+                nu_r[j, idx[sp]] += nu
+            for sp, nu in r.products.items():
+                nu_p[j, idx[sp]] += nu
+    A1 = torch.from_numpy(np.concatenate([nu_r, nu_p])).to(dev)            # [2 Nr, Ns]
+    A2 = torch.from_numpy(np.ascontiguousarray((nu_p - nu_r).T)).to(dev)   # [Ns, Nr]
+    g = torch.Generator(device=dev).manual_seed(1)
+    lnC = torch.randn(ns, a.ng, dtype=torch.float64, device=dev, generator=g)
+    q = torch.randn(nr, a.ng, dtype=torch.float64, device=dev, generator=g)
+    L = torch.empty(2 * nr, a.ng, dtype=torch.float64, device=dev)
+    W = torch.empty(ns, a.ng, dtype=torch.float64, device=dev)
+
+    def both():
+        torch.matmul(A1, lnC, out=L)
+        torch.matmul(A2, q, out=W)
+    _warm(both, a.warm)
+    ms_gemm = _time_ms(both, a.reps)
+    flops = 2.0 * (2 * nr * ns + ns * nr) * a.ng
+    out = dict(what="gemm", mech=name, ns=ns, nr=nr, ng=a.ng, gemm_ms=ms_gemm,
+               gemm_tflops=flops / ms_gemm * 1e-9, gemm_ns_per_point=ms_gemm * 1e6 / a.ng)
+    if m is not None:
+        v = best_variant(m)
+        dm = DeviceMechanism(m, v, build=False)
+        P, T, Y = random_states(ns, a.ng, seed=3, T_range=(a.tlo, a.thi))
+        Wk = torch.empty_like(Y)
+        _warm(lambda: dm.getrates_(P, T, Y, Wk), a.warm)
+        ms_k = _time_ms(lambda: dm.getrates_(P, T, Y, Wk), a.reps)
+        out.update(kernel=v.tag, getrates_ms=ms_k, getrates_ns_per_point=ms_k * 1e6 / a.ng,
+                   gemm_over_getrates=ms_gemm / ms_k)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
