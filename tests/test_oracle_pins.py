@@ -5,7 +5,8 @@
 * Y -> C against the ideal-gas law (sum C = P/(R0 T), mole fractions).
 * Arrhenius temperature dependence with Ea in cal/mol.
 * Detailed balance: with concentrations at equilibrium, built from Kp = exp(-dG/RT) with
-  H and S evaluated separately, the net rate vanishes, for dnu = -1, 0, +1.
+  H and S evaluated separately, the net rate vanishes, for dnu = -1, 0, +1, +0.5 (a
+  coefficient 0.5) and a side of four species with one species on both sides.
 * Third-body M: all efficiencies one -> M = P/(R0 T); an efficiency zero removes C_k.
 * Troe/Lindemann/SRI special cases with closed forms (Fcent = a, f1 = 0 -> F = a; F = 1;
   X = 1).
@@ -129,10 +130,12 @@ def _kc(species, nu, T):
     ({"H": 1.0, "O2": 1.0}, {"O": 1.0, "OH": 1.0}, "standard"),  # dnu = 0
     ({"H2O": 1.0}, {"H": 1.0, "OH": 1.0}, "third_body"),        # dnu = +1
     ({"H2O2": 1.0}, {"OH": 2.0}, "standard"),                   # nu = 2
+    ({"H2O2": 0.5}, {"OH": 1.0}, "standard"),                   # nu = 0.5, dnu = +0.5
+    ({"H": 1.0, "O2": 1.0, "H2": 1.0, "O": 1.0}, {"HO2": 1.0, "H": 1.0, "OH": 1.0}, "third_body"),  # 4 slots
 ])
 @pytest.mark.parametrize("T", [600.0, 1200.0, 2400.0])
 def test_detailed_balance(reac, prod, rtype, T):
-    names = ["H", "O", "OH", "H2O", "O2", "HO2", "H2O2", "N2"]
+    names = ["H", "O", "OH", "H2O", "O2", "HO2", "H2O2", "H2", "N2"]
     r = Reaction("", reac, prod, rtype, True, (2.0e13, 0.3, 5000.0))
     m = _mech(names, [r])
     idx = {n: i for i, n in enumerate(names)}
