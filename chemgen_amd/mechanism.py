@@ -174,11 +174,22 @@ class Mechanism:
 # ----------------------------------------------------------------------------------
 
 _COEF_RE = re.compile(r"^(\d+\.\d*|\.\d+|\d+)\s*(.+)$")
+# a coefficient in exponent form ("5.00E-01 O2", written by Cantera's converters) must be
+# followed by white space, so that a species name that starts with E is not read as one
+_COEF_EXP_RE = re.compile(r"^((?:\d+\.?\d*|\.\d+)[eE][-+]?\d+)\s+(.+)$")
 _FALLOFF_RE = re.compile(r"\(\s*\+\s*([^()\s]+(?:\([^()]*\))?)\s*\)")
 
 
 def _split_terms(side: str, species: set) -> List[str]:
     raw = [t.strip() for t in side.split("+")]
+    # re-join a coefficient that a "+" of its exponent split ("5.0E+01 O2")
+    joined: List[str] = []
+    for t in raw:
+        if joined and re.fullmatch(r"(?:\d+\.?\d*|\.\d+)[eE]", joined[-1]) and re.match(r"\d", t):
+            joined[-1] = joined[-1] + "+" + t
+        else:
+            joined.append(t)
+    raw = joined
     out: List[str] = []
     for t in raw:
         if t == "" and out and (out[-1] + "+") in species:
@@ -202,7 +213,9 @@ def _parse_side(side: str, species: set) -> Tuple[Dict[str, float], bool]:
         if term in species:
             name, nu = term, 1.0
         else:
-            m = _COEF_RE.match(term)
+            m = _COEF_EXP_RE.match(term)
+            if not m or m.group(2).strip() not in species:
+                m = _COEF_RE.match(term)
             if not m or m.group(2).strip() not in species:
                 raise ValueError(f"unknown species term '{term}' in '{side}'")
             name, nu = m.group(2).strip(), float(m.group(1))
@@ -622,6 +635,10 @@ def _qty_value(v, default_unit: str, table: Dict[str, float]) -> float:
 
 
 def load_yaml(path: str, compat: str = "full", name: Optional[str] = None) -> Mechanism:
+    """A Cantera YAML mechanism.  Non-stoichiometric reaction ``orders`` are not read: the
+    reference raises concentrations to the stoichiometric coefficients (Cantera's
+    ``reaction.reactants``, chemgen/parser.py:73-75, reactions/standard.j2:20) and converts A
+    with the same coefficients, and so does this build."""
     import yaml
     with open(path) as fh:
         doc = yaml.load(fh, Loader=_yaml_loader())

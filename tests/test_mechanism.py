@@ -25,6 +25,14 @@ def test_equation_parser_forms():
     assert coll == "AR"
     r, p, rev, m, coll = parse_equation("H2O+H2O=H+OH+H2O", sp | {"H2O"})
     assert r == {"H2O": 2.0} and p == {"H": 1.0, "OH": 1.0, "H2O": 1.0}
+    # coefficients as Cantera's converters write them (ck_files/CH4_Methane/TwoStep/chem.yaml)
+    r, p, rev, m, coll = parse_equation("CO + 5.00E-01 O2 <=> CO2", sp | {"CO", "CO2"})
+    assert r == {"CO": 1.0, "O2": 0.5} and p == {"CO2": 1.0} and rev
+    r, p, rev, m, coll = parse_equation("2.5e+00 O2 + 0.5 H => 1.5E1 O", sp)
+    assert r == {"O2": 2.5, "H": 0.5} and p == {"O": 15.0} and not rev
+    # a species whose name reads like an exponent stays a species
+    r, p, rev, m, coll = parse_equation("2E2 + O => OH", sp | {"E2"})
+    assert r == {"E2": 2.0, "O": 1.0}
 
 
 def test_molecular_weights_cantera_table():
@@ -50,6 +58,11 @@ def test_bundled_counts():
     j = c["synth_jetsurf348"]
     assert (j.n_species, j.n_reactions) == (348, 2163)
     assert j.counts()["troe"] + j.counts()["sri"] == 173
+    f = load_bundled("fuller2021")
+    assert (f.n_species, f.n_reactions) == (835, 4289)
+    assert f.counts() == {"standard": 3658, "third_body": 35, "troe": 63, "lindemann": 10, "plog": 523}
+    e = load_bundled("synth_ext53")
+    assert (e.n_species, e.n_reactions) == (53, 132)
 
 
 @pytest.mark.parametrize("name", ["synth_gri53", "synth_nhept88", "synth_jetsurf348"])
