@@ -637,7 +637,7 @@ def _qty_value(v, default_unit: str, table: Dict[str, float]) -> float:
 def load_yaml(path: str, compat: str = "full", name: Optional[str] = None) -> Mechanism:
     """A Cantera YAML mechanism.  Non-stoichiometric reaction ``orders`` are not read: the
     reference raises concentrations to the stoichiometric coefficients (Cantera's
-    ``reaction.reactants``, chemgen/parser.py:73-75, reactions/standard.j2:20) and converts A
+    ``reaction.reactants``, chemgen/parser.py:73-75, reactions/standard.j2:19) and converts A
     with the same coefficients, and so does this build."""
     import yaml
     with open(path) as fh:
