@@ -87,13 +87,18 @@ def test_ragged_batch_matches_oracle(name, grid):
 @pytest.mark.parametrize("name,tags", [
     ("synth_gri53", ("b64w1lghh", "b64w1lghhko", "s4w2lghhg8", "s4w3lghhg2", "s8w4lghhg4")),
     ("gri30", ("t16s8w2lh", "b64w2lghhL4")),
+    ("synth_ext53", ("t16s4w1lh", "t4s8w2lh")),
+    ("fuller2021", ("t8s16w4lh", "t2s8w2lh", "t1s4w1lh")),
 ])
 def test_ragged_batch_kernel_families(name, tags):
     """Both batched kernels (persistent walk, one workgroup per tile) of each kernel family
     -- lane, live-range lane with compiled-in coefficients, reaction-split (also with the
-    compact LDS layout), table, EG in LDS -- against the oracle on ragged patches."""
+    compact LDS layout), table (also with PLOG and extended records, and with the 1- to
+    8-point tiles of an 835-species mechanism), EG in LDS -- against the oracle on ragged
+    patches."""
     import torch
     from chemgen_amd.autotune import candidate_variants
+    from chemgen_amd.kernels import default_variants
     from chemgen_amd.runtime import DeviceMechanism
     m = load_bundled(name)
     sizes = np.array([130, 0, 64, 1, 250, 63, 65])
@@ -103,7 +108,7 @@ def test_ragged_batch_kernel_families(name, tags):
     ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
     d = torch.device("cuda:0")
     Pd, Td, Yd = (torch.from_numpy(x).to(d) for x in (P, T, Y))
-    by_tag = {v.tag: v for v in candidate_variants(m)}
+    by_tag = {v.tag: v for v in candidate_variants(m) + default_variants(m)}
     for tag in tags:
         dm = DeviceMechanism(m, by_tag[tag], build=False)
         for grid in (3, -1):
