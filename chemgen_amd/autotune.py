@@ -74,7 +74,7 @@ def table_supported(mech: Mechanism) -> bool:
     PLOG, at most 15 collision efficiencies per reaction, ...)."""
     from .codegen import table_records
     try:
-        table_records(mech)
+        table_records(mech, npt=1)
     except ValueError:
         return False
     return True

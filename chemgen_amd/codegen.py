@@ -952,10 +952,10 @@ class TableLayout:
 def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLayout:
     """The table kernel's reaction table for ``npt``-point tiles.  Each side of a reaction is
     written as up to three species *slots* (a species with coefficient nu fills nu slots),
-    absent slots point at an extra LDS row that holds EG = h = 1; slots are stored as LDS row
-    offsets (species index x npt).
-      row 0: x = blob offset of the reaction's record, y = r0 | r1 << 16, z = r2 | p0 << 16,
-             w = p1 | p2 << 16
+    absent slots point at an extra LDS row that holds EG = h = 1; slots are stored as LDS byte
+    offsets of the rows (species index x npt x 8; below 2^16 whenever the tile fits the LDS).
+      row 0: x = byte offset of the reaction's record in the blob, y = r0 | r1 << 16,
+             z = r2 | p0 << 16, w = p1 | p2 << 16
       row 1: x = kind | reversible << 3 | reactant slots << 4 | product slots << 6
                  | efficiencies << 8 | extended << 12 | (dnu + 8) << 24,
              y = offset of the collision-efficiency species in the efficiency list (PLOG: of
@@ -965,15 +965,16 @@ def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLay
     kind: 0 elementary, 1 third-body, 2 Lindemann, 3 Troe, 4 Troe with T2, 5 SRI, 6 SRI with
     e, 7 PLOG.  A reaction with a side of more than three slots or a non-integer coefficient
     is an extended record: its row-0 slots are unused (the padding row), and xsl[w:] holds the
-    number of reactant and of product entries and then each entry's LDS row offset, xnu[w:]
+    number of reactant and of product entries and then each entry's LDS row offset in doubles
+    (species index x npt), xnu[w:]
     its dnu (sum of product minus reactant coefficients), 0 and each entry's coefficient.
     Raises ValueError for what the format does not hold (more than 15 efficiencies, an LDS
     row offset beyond 16 bits)."""
     coef, off_smh, roffs = build_coef(mech)
     idx = {s.name: i for i, s in enumerate(mech.species)}
     pad = mech.n_species
-    if (pad + 1) * npt >= 1 << 16:
-        raise ValueError("table kernel: LDS row offsets exceed 16 bits")
+    if pad * npt * 8 >= 1 << 16:
+        raise ValueError("table kernel: LDS row byte offsets exceed 16 bits")
     for r in mech.reactions:
         if r.rtype not in _TABLE_KIND:
             raise ValueError(f"table kernel: reaction type {r.rtype} not supported")
@@ -991,7 +992,7 @@ def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLay
         lng = _table_long(r)
         if lng:
             nrs = nps = dnu = 0
-            rr = pp = [pad * npt] * 3
+            rr = pp = [8 * pad * npt] * 3
             xo = len(xsl)
             xsl += [len(r.reactants), len(r.products)]
             xsl += [idx[sp] * npt for sp in r.reactants] + [idx[sp] * npt for sp in r.products]
@@ -1001,12 +1002,12 @@ def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLay
             rsl, psl = _table_slots(r, idx)
             nrs, nps = len(rsl), len(psl)
             dnu = nps - nrs
-            rr = [k * npt for k in rsl] + [pad * npt] * (3 - nrs)
-            pp = [k * npt for k in psl] + [pad * npt] * (3 - nps)
+            rr = [8 * k * npt for k in rsl] + [8 * pad * npt] * (3 - nrs)
+            pp = [8 * k * npt for k in psl] + [8 * pad * npt] * (3 - nps)
             xo = 0
         x = kind | (int(r.reversible) << 3) | (nrs << 4) | (nps << 6) | (len(r.third_body) << 8) \
             | (TABLE_LONG if lng else 0) | ((dnu + 8) << 24)
-        rows.append((roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16), pp[1] | (pp[2] << 16)))
+        rows.append((8 * roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16), pp[1] | (pp[2] << 16)))
         if r.rtype == "plog":
             pts = sorted(set(c[0] for c in r.plog))
             y, o = len(plog), roffs[j]

@@ -14,13 +14,14 @@ from chemgen_amd.kernels import load_bundled
 @pytest.mark.parametrize("npt,min_class", [(16, 64), (64, 0), (16, 1000), (2, 8)])
 def test_table_decodes_to_mechanism(name, npt, min_class):
     m = load_bundled(name)
-    if (m.n_species + 1) * npt >= 1 << 16:
-        pytest.skip("LDS row offsets beyond 16 bits")
+    if m.n_species * npt * 8 >= 1 << 16:
+        pytest.skip("LDS row byte offsets beyond 16 bits")
     lay = table_layout(m, npt, min_class)
     rows, effsp, order, segs, gen_start = lay.rows, lay.effsp, lay.order, lay.segs, lay.gen_start
     assert table_records(m, npt, min_class) == (rows, effsp, order, segs, gen_start)
     _, _, roffs = build_coef(m)
-    pad = m.n_species * npt
+    pad = m.n_species * npt * 8       # slots: LDS byte offsets of the species rows
+    rowb = npt * 8
     idx = {s.name: i for i, s in enumerate(m.species)}
     assert sorted(order) == list(range(m.n_reactions))
     kinds = {"standard": {0}, "third_body": {1}, "lindemann": {2}, "troe": {3, 4}, "sri": {5, 6}, "plog": {7}}
@@ -32,7 +33,7 @@ def test_table_decodes_to_mechanism(name, npt, min_class):
         assert ((x >> 3) & 1) == int(r.reversible)
         nrs, nps, ne, dnu = (x >> 4) & 3, (x >> 6) & 3, (x >> 8) & 15, ((x >> 24) & 15) - 8
         lng = bool(x & TABLE_LONG)
-        assert y == roffs[j]
+        assert y == 8 * roffs[j]
         slots_r = [z & 0xFFFF, z >> 16, w & 0xFFFF]
         slots_p = [w >> 16, x2 & 0xFFFF, x2 >> 16]
         if lng:
@@ -50,9 +51,9 @@ def test_table_decodes_to_mechanism(name, npt, min_class):
         else:
             assert xo == 0
             for slots, count, side in ((slots_r, nrs, r.reactants), (slots_p, nps, r.products)):
-                assert all(s % npt == 0 for s in slots)
+                assert all(s % rowb == 0 for s in slots)
                 assert slots[count:] == [pad] * (3 - count)
-                assert Counter(s // npt for s in slots[:count]) == Counter({idx[k]: int(v) for k, v in side.items()})
+                assert Counter(s // rowb for s in slots[:count]) == Counter({idx[k]: int(v) for k, v in side.items()})
             assert dnu == int(r.coeff_sum)
         assert ne == len(r.third_body)
         if r.rtype == "plog":
