@@ -7,6 +7,7 @@ the ragged-batch launch.
     python -m chemgen_amd.profile_run batched --mech synth_nhept88 --reps 5
     python -m chemgen_amd.profile_run gemm --mech fuller2021 --ng 65536 --reps 5
     python -m chemgen_amd.profile_run gemm --shape 3761 16532 --ng 16384 --reps 5
+    python -m chemgen_amd.profile_run host --mech synth_gri53 --ng 1048576 --reps 5
 
 ``gemm`` times the GEMM formulation of a mechanism's stoichiometric products -- the logs of
 the forward and reverse rate-of-progress products, [2 Nr x Ns] x log C [Ns x ng], and the
@@ -14,6 +15,10 @@ production rates, nu [Ns x Nr] x q [Nr x ng] -- as dense FP64 GEMMs (torch.matmu
 on the FP64 matrix cores), and, for a bundled mechanism, the tuned getrates kernel at the
 same ng; it prints one JSON line with both times per point.  ``--shape NS NR`` takes a
 mechanism of that size with two reactants and two products per reaction.
+
+``host`` times ``GetRates(mech)(P, T, Y)`` called with NumPy arrays (the generated module's
+host-buffer form: copies to the GPU, the kernel, the copy back) in both Y layouts, against the
+kernel alone on device-resident data, and prints one JSON line.
 
 States are drawn on the GPU (T U(800, 2800) K unless --tlo/--thi, P 1-40 atm, Y Dirichlet(0.5)).
 """
@@ -41,7 +46,7 @@ def main(argv=None):
     from .runtime import DeviceMechanism
     from .surrogate import Surrogate, random_states
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["getrates", "infer", "batched", "gemm"])
+    ap.add_argument("what", choices=["getrates", "infer", "batched", "gemm", "host"])
     ap.add_argument("--mech", default="synth_gri53")
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--reps", type=int, default=5)
@@ -55,6 +60,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.what == "gemm":
         return _gemm(a)
+    if a.what == "host":
+        return _host(a)
     m = load_bundled(a.mech)
     if a.what == "batched":
         from .synth import ragged_patches
@@ -148,6 +155,40 @@ def _gemm(a):
         ms_k = _time_ms(lambda: dm.getrates_(P, T, Y, Wk), a.reps)
         out.update(kernel=v.tag, getrates_ms=ms_k, getrates_ns_per_point=ms_k * 1e6 / a.ng,
                    gemm_over_getrates=ms_gemm / ms_k)
+    print(json.dumps(out), flush=True)
+
+
+def _host(a):
+    """The host-buffer call (PCIe copies included) against the device-resident kernel."""
+    import json
+    import time
+    import numpy as np
+    import torch
+    from .api import GetRates
+    from .autotune import _time_ms
+    from .kernels import load_bundled
+    from .synth import make_states
+    m = load_bundled(a.mech)
+    g = GetRates(m, build=False)
+    P, T, Y = make_states(m.n_species, a.ng, seed=3, T_range=(a.tlo, a.thi))   # Y [Ns, ng]
+    Yp = np.ascontiguousarray(Y.T)                                           # Y [ng, Ns]
+    out = dict(what="host", mech=a.mech, ng=a.ng, kernel=g._dm.variant.tag)
+    for layout, Yh in (("species", Y), ("point", Yp)):
+        g(P, T, Yh, layout=layout)                       # first call: allocations, clocks
+        best = float("inf")
+        for _ in range(a.reps):
+            t0 = time.perf_counter()
+            g(P, T, Yh, layout=layout)
+            best = min(best, time.perf_counter() - t0)
+        out[f"{layout}_ms"] = best * 1e3
+        out[f"{layout}_points_per_s"] = a.ng / best
+    dev = torch.device("cuda:0")
+    Pd, Td, Yd = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (P, T, Y))
+    Wd = torch.empty_like(Yd)
+    _warm(lambda: g._dm.getrates_(Pd, Td, Yd, Wd), 1.0)
+    ms = _time_ms(lambda: g._dm.getrates_(Pd, Td, Yd, Wd), a.reps)
+    out.update(device_ms=ms, device_points_per_s=a.ng / (ms / 1e3),
+               host_bytes=int((m.n_species * 2 + 2) * 8 * a.ng))
     print(json.dumps(out), flush=True)
 
 
