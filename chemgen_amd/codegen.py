@@ -129,12 +129,19 @@ def _lk(off: int, beta: float, ea: float) -> str:
     return f"cf[{off}]" if terms is None else f"({terms} + cf[{off}])"
 
 
-def _arr_vals(A: float, beta: float, Ea: float) -> Tuple[float, float, float]:
-    """(log2 A, beta, Ea / (Rc ln 2)); A = 0 (a placeholder rate, kf = 0) is stored as
-    log2 A = -1100, which exp2 takes to 0."""
-    if A < 0.0:
-        raise ValueError(f"Arrhenius prefactor {A} < 0 outside a PLOG sum")
-    return (math.log2(r15(A)) if A > 0.0 else -1100.0, float(beta), r15(Ea) / RC * LOG2E)
+def _arr_vals(A: float, beta: float, Ea: float, signed: bool = False) -> Tuple[float, float, float]:
+    """(log2 |A|, beta, Ea / (Rc ln 2)); A = 0 (a placeholder rate, kf = 0) is stored as
+    log2 A = -1100, which exp2 takes to 0.  A < 0 (one of a pair of duplicate elementary
+    reactions whose rates are summed) is allowed where ``signed``: the sign is part of the
+    mechanism structure and compiled into the kernels (``negative_a``)."""
+    if A < 0.0 and not signed:
+        raise ValueError(f"Arrhenius prefactor {A} < 0 in a falloff reaction")
+    return (math.log2(r15(abs(A))) if A != 0.0 else -1100.0, float(beta), r15(Ea) / RC * LOG2E)
+
+
+def negative_a(r: Reaction) -> bool:
+    """An elementary or third-body reaction with a negative prefactor: kf = -2^(log2|A| + ...)."""
+    return r.rtype in ("standard", "third_body") and r.arrh[0] < 0.0
 
 
 def structure_key(mech: Mechanism) -> str:
@@ -155,7 +162,7 @@ def structure_key(mech: Mechanism) -> str:
             None if r.sri is None else len(r.sri),
             None if r.plog is None else tuple(len([c for c in r.plog if c[0] == p])
                                               for p in sorted(set(c[0] for c in r.plog))),
-        )))
+        ) + (("negative A",) if negative_a(r) else ())))
     return "\n".join(parts)
 
 
@@ -180,7 +187,7 @@ def build_coef(mech: Mechanism):
     roffs: List[int] = []
     for r in mech.reactions:
         if r.rtype in ("standard", "third_body"):
-            off = cb.add(*_arr_vals(*r.arrh))
+            off = cb.add(*_arr_vals(*r.arrh, signed=True))
         elif r.rtype in ("troe", "lindemann", "sri"):
             off = cb.add(*_arr_vals(*r.arrh))
             cb.add(*_arr_vals(*r.low))
@@ -354,13 +361,14 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
         dg = dg[2:]
     tb_off = None
     if r.rtype in ("standard", "third_body"):
+        sg = "-" if negative_a(r) else ""
         L.append(f"    const double lk = {_lk(off, r.arrh[1], r.arrh[2])};")
-        L.append("    const double kf = cg_exp2_fin(lk);")
+        L.append(f"    const double kf = {sg}cg_exp2_fin(lk);")
         if r.reversible and kr_mode != "gh":
             if kr_mode == "eg":
                 L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
             else:
-                L.append(f"    const double kr = cg_exp2(lk - ({dg}));")
+                L.append(f"    const double kr = {sg}cg_exp2(lk - ({dg}));")
         tb_off = off + 3
     elif r.rtype in ("troe", "lindemann", "sri"):
         nx = {"troe": 4 + (1 if r.troe and len(r.troe) == 4 else 0), "lindemann": 0, "sri": 5}[r.rtype]
@@ -860,6 +868,7 @@ def _emit_split(mech: Mechanism, variant: Variant):
 
 _TABLE_KIND = {"standard": 0, "third_body": 1, "lindemann": 2, "troe": 3, "sri": 5, "plog": 7}
 TABLE_LONG = 1 << 12      # class bit of a reaction written as an extended record
+TABLE_NEG = 1 << 13       # class bit of a reaction with a negative prefactor (kf < 0)
 
 
 def _table_long(r: Reaction) -> bool:
@@ -896,18 +905,20 @@ def _table_kind(r: Reaction) -> int:
     return kind
 
 
-def table_class(mech: Mechanism, j: int) -> Tuple[int, int, int, int, int]:
+def table_class(mech: Mechanism, j: int) -> Tuple[int, int, int, int, int, int]:
     """The shape the table kernel compiles a loop for: (family, reversible, reactant slots,
-    product slots, dnu if reversible else 0); family 0 elementary, 1 third-body,
-    2 Lindemann, 3 Troe (with or without T2), 5 SRI (with or without e), 7 PLOG.  A reaction
-    that needs an extended record has slot counts 0 (it never gets a loop of its own)."""
+    product slots, dnu if reversible else 0, negative prefactor); family 0 elementary,
+    1 third-body, 2 Lindemann, 3 Troe (with or without T2), 5 SRI (with or without e), 7 PLOG.
+    A reaction that needs an extended record has slot counts 0; neither it nor one with a
+    negative prefactor ever gets a loop of its own."""
     r = mech.reactions[j]
     fam = _TABLE_KIND[r.rtype]
+    neg = int(negative_a(r))
     if _table_long(r):
-        return (fam, int(r.reversible), 0, 0, 0)
+        return (fam, int(r.reversible), 0, 0, 0, neg)
     idx = {s.name: i for i, s in enumerate(mech.species)}
     rs, ps = _table_slots(r, idx)
-    return (fam, int(r.reversible), len(rs), len(ps), (len(ps) - len(rs)) if r.reversible else 0)
+    return (fam, int(r.reversible), len(rs), len(ps), (len(ps) - len(rs)) if r.reversible else 0, neg)
 
 
 def table_order(mech: Mechanism, min_class: int = 0):
@@ -922,13 +933,14 @@ def table_order(mech: Mechanism, min_class: int = 0):
         by.setdefault(table_class(mech, j), []).append(j)
     order: List[int] = []
     segs = []
-    compiled = lambda key: key[2] > 0 and len(by[key]) >= max(1, min_class)
+    compiled = lambda key: key[2] > 0 and not key[5] and len(by[key]) >= max(1, min_class)
     for key in sorted(by):
         if compiled(key):
             segs.append((len(order), len(order) + len(by[key]), key))
             order.extend(by[key])
     rest = sorted((j for key in by if not compiled(key) for j in by[key]),
-                  key=lambda j: (_table_long(mech.reactions[j]), _table_kind(mech.reactions[j]),
+                  key=lambda j: (_table_long(mech.reactions[j]), negative_a(mech.reactions[j]),
+                                 _table_kind(mech.reactions[j]),
                                  mech.reactions[j].reversible, table_class(mech, j)[2:4],
                                  len(mech.reactions[j].third_body), j))
     gen_start = len(order)
@@ -957,7 +969,8 @@ def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLay
       row 0: x = byte offset of the reaction's record in the blob, y = r0 | r1 << 16,
              z = r2 | p0 << 16, w = p1 | p2 << 16
       row 1: x = kind | reversible << 3 | reactant slots << 4 | product slots << 6
-                 | efficiencies << 8 | extended << 12 | (dnu + 8) << 24,
+                 | efficiencies << 8 | extended << 12 | negative prefactor << 13
+                 | (dnu + 8) << 24,
              y = offset of the collision-efficiency species in the efficiency list (PLOG: of
              its pressure points in the PLOG list), z = blob offset of their (eff - 1) values
              (PLOG: the number of pressure points), w = offset of an extended record in
@@ -1006,7 +1019,7 @@ def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLay
             pp = [8 * k * npt for k in psl] + [8 * pad * npt] * (3 - nps)
             xo = 0
         x = kind | (int(r.reversible) << 3) | (nrs << 4) | (nps << 6) | (len(r.third_body) << 8) \
-            | (TABLE_LONG if lng else 0) | ((dnu + 8) << 24)
+            | (TABLE_LONG if lng else 0) | (TABLE_NEG if negative_a(r) else 0) | ((dnu + 8) << 24)
         rows.append((8 * roffs[j], rr[0] | (rr[1] << 16), rr[2] | (pp[0] << 16), pp[1] | (pp[2] << 16)))
         if r.rtype == "plog":
             pts = sorted(set(c[0] for c in r.plog))
@@ -1046,7 +1059,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
     lay = table_layout(mech, npt, min_class=max(1, slots // 2))
     rows, effsp, segs, gen_start = lay.rows, lay.effsp, lay.segs, lay.gen_start
     classes = []
-    for (a, b, (fam, rev, nrs, nps, dnu)) in segs:
+    for (a, b, (fam, rev, nrs, nps, dnu, _neg)) in segs:
         pf = None
         if rev:
             pf = {0: None, 1: "pinv", -1: "pfac"}.get(dnu, f"cg_exp2({dlit(-float(dnu))} * lg2pfac)")
@@ -1067,6 +1080,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
         classes=classes, gen_start=gen_start,
         has_plog=bool(lay.plog), plog=", ".join(f"{{{o}, {n}}}" for o, n in lay.plog), n_plog=len(lay.plog),
         has_long=len(lay.xsl) > 2, xsl=", ".join(str(v) for v in lay.xsl), n_xsl=len(lay.xsl),
+        has_neg=any(negative_a(r) for r in mech.reactions),
         xnu=", ".join(dlit(v) for v in lay.xnu),
         tmid=", ".join(dlit(s.thermo.t_mid) for s in mech.species),
     )

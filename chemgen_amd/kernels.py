@@ -20,12 +20,14 @@ def load_bundled(name: str) -> Mechanism:
 TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348", "h2_burke", "fuller2021")
 
 # table-kernel shapes compiled for small mechanisms whose tuned kernel is another family, so
-# that the table format's PLOG and extended records are exercised: 16-point tiles of 4
+# that the table format's PLOG, extended and negative-prefactor records are exercised: 16-point tiles of 4
 # wavefronts (compiled class loops and the generic loop) and 4-point tiles of 8
 TABLE_CHECKED = {
     "synth_plog53": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 16),
                      Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
     "synth_ext53": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 16),
+                    Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
+    "hashemi2016": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 16),
                     Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
 }
 

@@ -39,6 +39,7 @@ def _run(torch, mech, P, T, Y, dm=None):
     ("synth_plog53", 2048, (300.0, 2500.0)),
     ("synth_ext53", 2048, (300.0, 2500.0)),
     ("fuller2021", 256, (600.0, 2800.0)),
+    ("hashemi2016", 2048, (300.0, 2500.0)),
 ])
 def test_parity_vs_oracle(torch_cuda, name, ng, trange):
     m = load_bundled(name)
@@ -63,14 +64,15 @@ def test_ragged_tail_sizes(torch_cuda, ng):
 
 
 @pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53", "synth_ext53",
-                                  "fuller2021"])
+                                  "fuller2021", "hashemi2016"])
 def test_table_kernels_vs_oracle(torch_cuda, name):
     """Every table-driven kernel built for the mechanism (all tile lengths and wavefront counts
     that fit the LDS) against the oracle; gri30 brings Troe with and without T2 and Lindemann
     reactions, the JetSurf-sized mechanism SRI, synth_plog53 PLOG reactions (in compiled class
     loops and the generic loop), synth_ext53 extended records (sides of 4-5 species slots,
     coefficients 0.5, with PLOG, third-body and falloff rates), Fuller 2021 (835 species) all of
-    them in 1- to 8-point tiles; 300 points leave a partial tile for every tile length."""
+    them in 1- to 8-point tiles, Hashemi 2016 duplicate reactions with negative prefactors;
+    300 points leave a partial tile for every tile length."""
     from chemgen_amd.kernels import default_variants
     from chemgen_amd.runtime import DeviceMechanism
     m = load_bundled(name)
@@ -150,7 +152,7 @@ def _edge_states(m):
 
 
 @pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_nhept88", "synth_jetsurf348",
-                                  "synth_plog53", "synth_ext53", "fuller2021"])
+                                  "synth_plog53", "synth_ext53", "fuller2021", "hashemi2016"])
 def test_edge_states_vs_oracle(torch_cuda, name):
     """The tuned kernel of each mechanism on the crafted states of _edge_states.  Where the
     oracle itself overflows (Fuller 2021 at 3000 K and above: exp(smh) of species whose NASA-7

@@ -8,6 +8,7 @@
   H and S evaluated separately, the net rate vanishes, for dnu = -1, 0, +1, +0.5 (a
   coefficient 0.5) and a side of four species with one species on both sides.
 * Third-body M: all efficiencies one -> M = P/(R0 T); an efficiency zero removes C_k.
+* Duplicates with a negative prefactor add up like one reaction with the summed prefactor.
 * Troe/Lindemann/SRI special cases with closed forms (Fcent = a, f1 = 0 -> F = a; F = 1;
   X = 1).
 * Pressure scaling q ~ P^order at fixed T and composition.
@@ -155,6 +156,20 @@ def test_detailed_balance(reac, prod, rtype, T):
     P, Tt, Y = _state(m, C, T)
     w, qf, qb = getrates(m, P, Tt, Y, return_parts=True)
     assert abs(w[idx[lastp], 0]) <= 1e-12 * qf[0, 0] * lastnu
+
+
+def test_negative_prefactor_duplicates_sum():
+    """Duplicate reactions with prefactors A1 > 0 and A2 < 0 (same beta, Ea) rate as one reaction
+    with A1 + A2 (real mechanisms write deviations from Arrhenius form this way)."""
+    arr = lambda A: (A, 0.7, 8000.0)
+    pair = [Reaction("", {"H": 1.0, "O2": 1.0}, {"O": 1.0, "OH": 1.0}, "standard", True, arr(3.0e13)),
+            Reaction("", {"H": 1.0, "O2": 1.0}, {"O": 1.0, "OH": 1.0}, "standard", True, arr(-1.0e13))]
+    one = [Reaction("", {"H": 1.0, "O2": 1.0}, {"O": 1.0, "OH": 1.0}, "standard", True, arr(2.0e13))]
+    names = ["H", "O", "OH", "O2", "N2"]
+    P, T, Y = make_states(len(names), 64, seed=3)
+    w2 = getrates(_mech(names, pair), P, T, Y)
+    w1 = getrates(_mech(names, one), P, T, Y)
+    assert np.allclose(w2, w1, rtol=1e-13, atol=0.0)
 
 
 def test_third_body_efficiencies_closed_form():

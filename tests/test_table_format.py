@@ -5,12 +5,12 @@ from collections import Counter
 import pytest
 
 from chemgen_amd.autotune import table_supported
-from chemgen_amd.codegen import TABLE_LONG, build_coef, table_class, table_layout, table_records
+from chemgen_amd.codegen import TABLE_LONG, TABLE_NEG, build_coef, negative_a, table_class, table_layout, table_records
 from chemgen_amd.kernels import load_bundled
 
 
 @pytest.mark.parametrize("name", ["h2_burke", "gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53",
-                                  "synth_ext53", "fuller2021"])
+                                  "synth_ext53", "fuller2021", "hashemi2016"])
 @pytest.mark.parametrize("npt,min_class", [(16, 64), (64, 0), (16, 1000), (2, 8)])
 def test_table_decodes_to_mechanism(name, npt, min_class):
     m = load_bundled(name)
@@ -33,6 +33,7 @@ def test_table_decodes_to_mechanism(name, npt, min_class):
         assert ((x >> 3) & 1) == int(r.reversible)
         nrs, nps, ne, dnu = (x >> 4) & 3, (x >> 6) & 3, (x >> 8) & 15, ((x >> 24) & 15) - 8
         lng = bool(x & TABLE_LONG)
+        assert bool(x & TABLE_NEG) == negative_a(r)
         assert y == 8 * roffs[j]
         slots_r = [z & 0xFFFF, z >> 16, w & 0xFFFF]
         slots_p = [w >> 16, x2 & 0xFFFF, x2 >> 16]
@@ -76,7 +77,7 @@ def test_table_decodes_to_mechanism(name, npt, min_class):
     # extended records never get a loop of their own
     pos = 0
     for a, b, key in segs:
-        assert a == pos and b - a >= max(1, min_class) and key[2] > 0
+        assert a == pos and b - a >= max(1, min_class) and key[2] > 0 and not key[5]
         assert all(table_class(m, order[n]) == key for n in range(a, b))
         pos = b
     assert gen_start == pos
@@ -127,3 +128,24 @@ def test_live_order_is_a_permutation_with_its_peak(name):
                 lo[idx[s]] = pos
         file_peak = max(sum(1 for k in fo if fo[k] <= pos <= lo[k]) for pos in range(m.n_reactions))
         assert peak < file_peak
+
+
+def test_negative_prefactor_compiled_in():
+    """Negative prefactors (Hashemi 2016: 22 duplicates) keep log2 |A| in the blob and their sign
+    in the code: the straight-line kernel negates exactly those rates, the table kernel marks
+    them with a class bit and keeps them out of the compiled class loops."""
+    import numpy as np
+    from chemgen_amd.codegen import Variant, emit_hip
+    m = load_bundled("hashemi2016")
+    neg = [j for j, r in enumerate(m.reactions) if negative_a(r)]
+    assert len(neg) == 22
+    coef, _, roffs = build_coef(m)
+    assert np.isfinite(coef).all()
+    for j in neg:
+        assert coef[roffs[j]] == pytest.approx(np.log2(abs(m.reactions[j].arrh[0])), rel=1e-15)
+    src = emit_hip(m, Variant())[0]
+    assert src.count("const double kf = -cg_exp2_fin(lk);") == 22
+    lay = table_layout(m, 16, 8)
+    pos = {j: n for n, j in enumerate(lay.order)}
+    assert all(lay.rows[2 * pos[j] + 1][0] & TABLE_NEG for j in neg)
+    assert all(pos[j] >= lay.gen_start for j in neg)
