@@ -80,17 +80,34 @@ def table_supported(mech: Mechanism) -> bool:
     return True
 
 
+def straight_line_size(mech: Mechanism) -> int:
+    """Reactions a straight-line kernel spells out: one per reaction plus one per extra
+    Arrhenius term of a PLOG reaction (every term of every pressure point is compiled in)."""
+    return mech.n_reactions + sum(len(r.plog) - 1 for r in mech.reactions if r.rtype == "plog")
+
+
 def default_variant(mech: Mechanism) -> Variant:
-    """Shape used before any tuning: lane per point for small mechanisms, the table kernel
-    for large ones (8 wavefronts, the longest tile of at most 16 points whose species state
-    fits the LDS; 4 wavefronts when 8 do not fit)."""
-    if mech.n_species <= STRAIGHT_LINE_MAX_SPECIES:
+    """Shape used before any tuning, from the tuner's measurements on MI355X: lane per point for
+    small mechanisms (<= 24 species; QSS mechanisms always); reaction-split tiles up to 150
+    species while the straight-line code stays small (<= 800 reactions spelled out: GRI-sized
+    and n-heptane-sized tuned there, 2x the lane kernel), 4 wavefronts per tile up to 64
+    species (two tiles per CU), 8 above; otherwise the table kernel (a mechanism such as Hashemi
+    2016, 68 species with 114 PLOG reactions of 908 terms, spills 2-9 KB per lane in the
+    straight-line kernels and runs 1.4-3.6x faster in the table kernel), with 8 wavefronts and
+    the longest tile of at most 16 points whose species state fits the LDS (4 wavefronts when 8
+    do not fit)."""
+    ns = mech.n_species
+    if ns <= 24 or getattr(mech, "qss", ()):
         return Variant()
+    if ns <= STRAIGHT_LINE_MAX_SPECIES and straight_line_size(mech) <= 800:
+        if ns <= 64:
+            return Variant(64, 2, "lean", "gh", "horner", False, 28, 4)
+        return Variant(64, 2, "lean", "gh", "horner", False, 8, 8)
     for npt in (16, 8, 4, 2, 1):
         for waves, wpe in ((8, 2), (4, 1)):
-            if table_lds_bytes(mech.n_species, npt, waves) <= LDS_BYTES:
+            if table_lds_bytes(ns, npt, waves) <= LDS_BYTES:
                 return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt)
-    raise ValueError(f"{mech.n_species} species: the table kernel's one-point tile does not fit the LDS")
+    raise ValueError(f"{ns} species: the table kernel's one-point tile does not fit the LDS")
 
 
 def _load_db() -> Dict:
