@@ -19,8 +19,9 @@ Launch shape, the analogue of the reference's parallel levels (--parallel-level,
 --veclen, --nreact-per-block):
   1  one lane per grid point, VECLEN points per workgroup (rounded up to 64)
   2  reaction-split tiles: NREACT_PER_BLOCK wavefronts share a 64-point tile in LDS
-  3  table-driven tiles of VECLEN points (16, 32 or 64) and NREACT_PER_BLOCK wavefronts, each
-     lane one (point, reaction slot) pair -- the shape for mechanisms too large for 2
+  3  table-driven tiles of VECLEN points (1, 2, 4, 8, 16, 32 or 64: the nearest) and
+     NREACT_PER_BLOCK wavefronts, each lane one (point, reaction slot) pair -- the shape for
+     mechanisms too large for 2 (500+ species need tiles of 8 points or fewer)
   4  the autotuned shape for this mechanism (data/tuning.json), else the default
 
 ``--legacy-abi`` (with ``--compat reference``) also writes ``<stem>_legacy.so``: the
@@ -86,7 +87,7 @@ def variant_for(args, mech):
         return Variant(64, 2, "lean", "gh", "horner", False, 8, max(2, args.nreact_per_block))
     if lvl == 3:
         from .autotune import LDS_BYTES, table_lds_bytes
-        npt = min((16, 32, 64), key=lambda n: abs(n - args.veclen))
+        npt = min((1, 2, 4, 8, 16, 32, 64), key=lambda n: abs(n - args.veclen))
         waves = max(1, args.nreact_per_block)
         if table_lds_bytes(mech.n_species, npt, waves) > LDS_BYTES:
             raise SystemExit(f"--parallel-level 3: {npt} points x {mech.n_species} species exceed the LDS; "

@@ -149,3 +149,16 @@ def test_negative_prefactor_compiled_in():
     pos = {j: n for n, j in enumerate(lay.order)}
     assert all(lay.rows[2 * pos[j] + 1][0] & TABLE_NEG for j in neg)
     assert all(pos[j] >= lay.gen_start for j in neg)
+
+
+def test_generator_level3_short_tiles():
+    """--parallel-level 3 takes tiles down to one point (what 500+-species mechanisms need) and
+    refuses a tile whose species state does not fit the LDS."""
+    from argparse import Namespace
+    from chemgen_amd.generate_chemistry import variant_for
+    m = load_bundled("fuller2021")
+    v = variant_for(Namespace(parallel_level=3, veclen=8, nreact_per_block=16), m)
+    assert (v.table, v.split) == (8, 16)
+    assert variant_for(Namespace(parallel_level=3, veclen=1, nreact_per_block=4), m).table == 1
+    with pytest.raises(SystemExit):
+        variant_for(Namespace(parallel_level=3, veclen=16, nreact_per_block=16), m)
