@@ -146,10 +146,10 @@ def bench_batched(steps, warmup):
 
 
 def bench_large(steps, warmup, ng=1 << 19, name="synth_jetsurf348", nchk=256):
-    """Plain FP64 getrates on a large mechanism, T U(600, 2800) K, with the tuned
+    """Plain FP64 getrates on a large mechanism, T U(600, 2800) K, with its tuned or default
     (table-driven) kernel: the JetSurf-sized synthetic mechanism (348 species, 2163 reactions)
-    over 2^19 points, or Fuller et al. 2021 (835 species, 4289 reactions with 523 PLOG) over
-    2^18."""
+    over 2^19 points, Fuller et al. 2021 (835 species, 4289 reactions with 523 PLOG) over 2^18,
+    or Hashemi et al. 2016 (68 species, 631 reactions with 114 PLOG of 908 terms) over 2^20."""
     import numpy as np
     import torch
     from chemgen_amd.autotune import best_variant
@@ -384,7 +384,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the ragged-batch workload")
-    ap.add_argument("--no-large", action="store_true", help="skip the JetSurf-sized and Fuller 2021 workloads")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the JetSurf-sized, Fuller 2021 and Hashemi 2016 workloads")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate workload")
     ap.add_argument("--no-h2", action="store_true", help="skip the H2 Burke mini-app workload")
     ap.add_argument("--no-qss", action="store_true", help="skip the QSS-species workload")
@@ -489,6 +490,9 @@ def main():
         torch.cuda.empty_cache()
         out["large_real"] = bench_large(max(3, args.steps // 4), min(args.warmup, 2), ng=1 << 18,
                                         name="fuller2021", nchk=128)
+        torch.cuda.empty_cache()
+        out["plog_real"] = bench_large(max(3, args.steps // 4), min(args.warmup, 2), ng=1 << 20,
+                                       name="hashemi2016", nchk=256)
     if not args.no_surrogate:
         torch.cuda.empty_cache()
         out["surrogate"] = bench_surrogate(args.steps, args.warmup)
