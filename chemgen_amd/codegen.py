@@ -392,7 +392,8 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
     if r.rtype in ("troe", "lindemann", "sri"):
         L.append(f"    const double lki = {_lk(off, r.arrh[1], r.arrh[2])};")
         L.append(f"    const double lk0 = {_lk(off + 3, r.low[1], r.low[2])};")
-        L.append("    const double kinf = cg_exp2_fin(lki);")
+        if r.rtype == "lindemann":
+            L.append("    const double kinf = cg_exp2_fin(lki);")
         L.append("    const double pr = M * cg_exp2_fin(lk0 - lki);")
         o = off + 6
         if r.rtype == "troe":
@@ -407,16 +408,18 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
             L.append("    const double nn = 0.75 - 1.27 * lf;")
             # log10 F = log10 Fcent / (1 + (x/d)^2) = log10 Fcent d^2 / (d^2 + x^2): one division
             L.append("    const double x = lp + cc, d = nn - 0.14 * x;")
-            L.append("    const double F = cg_exp2(cg_div(l2f * (d * d), __builtin_fma(x, x, d * d)));")
-            L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr) * F;")
+            # k_inf F = 2^(log2 k_inf + log2 F): one exp2 for both
+            L.append("    const double l2F = cg_div(l2f * (d * d), __builtin_fma(x, x, d * d));")
+            L.append("    const double kf = cg_exp2(lki + l2F) * cg_div(pr, 1.0 + pr);")
         elif r.rtype == "sri":
             L.append(f"    const double lp = cg_log2(pr) * {dlit(LOG10_2)};")
             L.append("    const double X = cg_div(1.0, 1.0 + lp * lp);")
             L.append(f"    const double base = cf[{o}] * cg_exp2(-cf[{o + 1}] * invT) + cg_exp2(-t * cf[{o + 2}]);")
-            L.append(f"    double F = cf[{o + 3}] * cg_exp2(X * cg_log2(base));")
+            # k_inf F = d 2^(log2 k_inf + X log2 base [+ e log2 T]): one exp2 for all three
+            l2F = "X * cg_log2(base)"
             if len(r.sri) == 5 and r.sri[4] != 0.0:
-                L.append(f"    F *= cg_exp2(cf[{o + 4}] * lg2T);")
-            L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr) * F;")
+                l2F = f"__builtin_fma(cf[{o + 4}], lg2T, {l2F})"
+            L.append(f"    const double kf = cf[{o + 3}] * cg_exp2(lki + {l2F}) * cg_div(pr, 1.0 + pr);")
         else:
             L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr);")
         if r.reversible and kr_mode != "gh":
