@@ -12,6 +12,8 @@ Outputs (Chemkin text, thermo inline):
                         523 PLOG, 73 falloff, 35 third-body; 43 reaction sides of 4-5 species)
   hashemi2016.inp       ck_files/CH4_Methane/Hashemi2016/chem.yaml (68 species, 631 reactions:
                         114 PLOG, 34 falloff, 11 third-body; 22 with a negative prefactor)
+  ic8_llnl.inp          ck_files/i-Octane_C8H18/llnl_version3/chem.yaml (874 species, 6864
+                        reactions: the reference's largest species count)
   synth_gri53.inp       GRI30 species; 325 reactions: 25 Troe + 4 Lindemann, 40 third-body
   synth_nhept88.inp     n-heptane sk88 species; 387 reactions: 33 Troe + 6 Lindemann, 20 third-body
   synth_jetsurf348.inp  JetSurf 2.0 species; 2163 reactions: 150 Troe + 23 SRI, 324 third-body
@@ -39,6 +41,7 @@ def main(ck_files: str, only=()) -> None:
         "gri30": "GRI30/chem.yaml",
         "fuller2021": "Nitrogen/Fuller-2021/chem.yaml",
         "hashemi2016": "CH4_Methane/Hashemi2016/chem.yaml",
+        "ic8_llnl": "i-Octane_C8H18/llnl_version3/chem.yaml",
     }
     for name, rel in real.items():
         if only and name not in only:

@@ -40,6 +40,7 @@ def _run(torch, mech, P, T, Y, dm=None):
     ("synth_ext53", 2048, (300.0, 2500.0)),
     ("fuller2021", 256, (600.0, 2800.0)),
     ("hashemi2016", 2048, (300.0, 2500.0)),
+    ("ic8_llnl", 128, (600.0, 2800.0)),
 ])
 def test_parity_vs_oracle(torch_cuda, name, ng, trange):
     m = load_bundled(name)
