@@ -93,9 +93,9 @@ def default_variant(mech: Mechanism) -> Variant:
     and n-heptane-sized tuned there, 2x the lane kernel), 4 wavefronts per tile up to 64
     species (two tiles per CU), 8 above; otherwise the table kernel (a mechanism such as Hashemi
     2016, 68 species with 114 PLOG reactions of 908 terms, spills 2-9 KB per lane in the
-    straight-line kernels and runs 1.4-3.6x faster in the table kernel), with 8 wavefronts and
-    the longest tile of at most 16 points whose species state fits the LDS (4 wavefronts when 8
-    do not fit)."""
+    straight-line kernels and runs 1.4-3.6x faster in the table kernel), with the longest tile of
+    at most 16 points whose species state fits the LDS and the most wavefronts that fit with it
+    (16, 4 per SIMD, won at every tile length on the JetSurf-sized mechanism and Fuller 2021)."""
     ns = mech.n_species
     if ns <= 24 or getattr(mech, "qss", ()):
         return Variant()
@@ -104,7 +104,7 @@ def default_variant(mech: Mechanism) -> Variant:
             return Variant(64, 2, "lean", "gh", "horner", False, 28, 4)
         return Variant(64, 2, "lean", "gh", "horner", False, 8, 8)
     for npt in (16, 8, 4, 2, 1):
-        for waves, wpe in ((8, 2), (4, 1)):
+        for waves, wpe in ((16, 4), (8, 2), (4, 1)):
             if table_lds_bytes(ns, npt, waves) <= LDS_BYTES:
                 return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt)
     raise ValueError(f"{ns} species: the table kernel's one-point tile does not fit the LDS")
