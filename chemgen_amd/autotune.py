@@ -281,14 +281,20 @@ if __name__ == "__main__":
     import argparse
     from .kernels import load_bundled
     ap = argparse.ArgumentParser()
-    ap.add_argument("mechs", nargs="*", default=["synth_gri53"])
+    ap.add_argument("mechs", nargs="*", default=["synth_gri53"],
+                    help="bundled mechanism names or mechanism files (Chemkin with thermo inline, or YAML)")
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--out", default=None, help="where to write the tuning database")
+    ap.add_argument("--build", action="store_true", help="compile the candidates that are not built yet")
     ap.add_argument("--batched", action="store_true",
                     help="tune the ragged-batch launch: 64 patches, log-uniform 2^8-2^17 points, 2^22 in total")
     a = ap.parse_args()
     for name in a.mechs:
-        m = load_bundled(name)
+        if os.path.isfile(name):
+            from .mechanism import load_mechanism
+            m = load_mechanism(name, name=os.path.splitext(os.path.basename(name))[0])
+        else:
+            m = load_bundled(name)
         if a.batched:
             import torch
             from .synth import make_states, ragged_patches
@@ -299,4 +305,4 @@ if __name__ == "__main__":
             del P, T, Y
             tune_batched(m, offs, Pd, Td, Yd, out_path=a.out)
         else:
-            tune(m, ng=a.ng, out_path=a.out)
+            tune(m, ng=a.ng, out_path=a.out, build=a.build)
