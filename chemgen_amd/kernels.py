@@ -17,7 +17,7 @@ def load_bundled(name: str) -> Mechanism:
     return parse_chemkin(os.path.join(DATA_DIR, name + ".inp"), name=name)
 
 
-TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348", "h2_burke", "fuller2021")
+TUNED = ("synth_gri53", "synth_nhept88", "gri30", "synth_jetsurf348", "h2_burke", "fuller2021", "ic8_llnl")
 
 # table-kernel shapes compiled for small mechanisms whose tuned kernel is another family, so
 # that the table format's PLOG, extended and negative-prefactor records are exercised: 16-point tiles of 4

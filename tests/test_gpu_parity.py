@@ -65,7 +65,7 @@ def test_ragged_tail_sizes(torch_cuda, ng):
 
 
 @pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53", "synth_ext53",
-                                  "fuller2021", "hashemi2016"])
+                                  "fuller2021", "hashemi2016", "ic8_llnl"])
 def test_table_kernels_vs_oracle(torch_cuda, name):
     """Every table-driven kernel built for the mechanism (all tile lengths and wavefront counts
     that fit the LDS) against the oracle; gri30 brings Troe with and without T2 and Lindemann
