@@ -33,6 +33,23 @@ R0 = 8.314510e+07
 RC = 1.9872155832
 PATM = 1013250.0
 
+# Troe broadening constants (chemgen/templates/reactions/troe.j2:29-36; Gilbert, Luther & Troe,
+# Ber. Bunsenges. Phys. Chem. 87 (1983) 169): c = C0 + C1 log10 Fcent, n = N0 + N1 log10 Fcent,
+# f1 = (log10 Pr + c) / (n - D (log10 Pr + c)).  Module-level so that tests can plant mistakes.
+TROE_C0, TROE_C1 = -0.4, -0.67
+TROE_N0, TROE_N1 = 0.75, -1.27
+TROE_D = 0.14
+
+
+def troe_t2_term(T2, T):
+    """The optional third Fcent term exp(-T2/T) (troe.j2:29-31)."""
+    return np.exp(r15(-T2) / T)
+
+
+def sri_exponent(lp):
+    """SRI exponent X = 1 / (1 + log10(Pr)^2) (Kee, Coltrin & Glarborg, eq. 9.101)."""
+    return 1.0 / (1.0 + lp ** 2)
+
 
 def r15(x: float) -> float:
     """A coefficient as the templates print it: ``"%.15e" % x``."""
@@ -114,18 +131,18 @@ def _falloff_kf(r, T, M):
             t1 = np.exp(-T / r15(T1)) if T1 != 0.0 else np.zeros_like(T)
             Fcent = r15(1.0 - a) * t3 + r15(a) * t1
             if len(tp) == 4 and tp[3]:
-                Fcent = Fcent + np.exp(r15(-tp[3]) / T)
+                Fcent = Fcent + troe_t2_term(tp[3], T)
             lf = np.log10(Fcent)
-            C1 = -0.4 - 0.67 * lf
-            N1 = 0.75 - 1.27 * lf
+            C1 = TROE_C0 + TROE_C1 * lf
+            N1 = TROE_N0 + TROE_N1 * lf
             lp = np.log10(Pr)
-            f1 = (lp + C1) / (N1 - 0.14 * (lp + C1))
+            f1 = (lp + C1) / (N1 - TROE_D * (lp + C1))
             F = 10.0 ** (lf / (1.0 + f1 ** 2))
         elif r.rtype == "sri":
             sp = tuple(r.sri) + ((1.0, 0.0) if len(r.sri) == 3 else ())
             a, b_, c, d, e = sp
             lp = np.log10(Pr)
-            X = 1.0 / (1.0 + lp ** 2)
+            X = sri_exponent(lp)
             F = r15(d) * (r15(a) * np.exp(r15(-b_) / T) + np.exp(-T / r15(c))) ** X
             if e != 0.0:
                 F = F * np.power(T, float(e))

@@ -9,8 +9,9 @@
   coefficient 0.5) and a side of four species with one species on both sides.
 * Third-body M: all efficiencies one -> M = P/(R0 T); an efficiency zero removes C_k.
 * Duplicates with a negative prefactor add up like one reaction with the summed prefactor.
-* Troe/Lindemann/SRI special cases with closed forms (Fcent = a, f1 = 0 -> F = a; F = 1;
-  X = 1).
+* Troe/Lindemann/SRI special cases with closed forms: Fcent = a, f1 = 0 -> F = a; f1 = +-1 ->
+  F = Fcent^(1/2); n = 0 and the |log10 Pr| -> inf limit -> F = Fcent^(1/(1+(1/0.14)^2)); the T2
+  term; Lindemann F = 1; SRI X = 1, 1/2, 1/5, 1/10 at Pr = 1, 10^+-1, 10^+-2, 10^3.
 * Pressure scaling q ~ P^order at fixed T and composition.
 * Element and mass conservation for every bundled mechanism.
 * Planted mistakes in the oracle make the pins fail.
@@ -199,6 +200,90 @@ def test_troe_center_closed_form():
     assert q == pytest.approx(A * pr / (1 + pr) * a * C[0] * C[1], rel=1e-12)
 
 
+def _falloff_q(rtype, pr, T=1500.0, A=2.0e12, **kw):
+    """-wdot of H for one irreversible falloff reaction H + O2 (+M) => HO2 at reduced pressure
+    ``pr`` (M = ctot; k0 and kinf temperature-independent, so Pr = A0 ctot / A)."""
+    ctot = 1.0e-5
+    A0 = pr * A / ctot
+    r = _irrev({"H": 1.0, "O2": 1.0}, {"HO2": 1.0}, (A, 0.0, 0.0), rtype=rtype, low=(A0, 0.0, 0.0), **kw)
+    m = _mech(["H", "O2", "HO2", "N2"], [r])
+    C = np.array([0.1, 0.2, 0.0, 0.7]) * ctot
+    P, Tt, Y = _state(m, C, T)
+    return -getrates(m, P, Tt, Y)[0, 0], A * C[0] * C[1]
+
+
+# Troe broadening, textbook definition (Gilbert, Luther & Troe 1983; Kee, Coltrin & Glarborg,
+# Chemically Reacting Flow, eq. 9.99): log10 F = log10 Fcent / (1 + f1^2),
+# f1 = (log10 Pr + c) / (n - 0.14 (log10 Pr + c)), c = -0.4 - 0.67 log10 Fcent,
+# n = 0.75 - 1.27 log10 Fcent.
+def _troe_lp_for_f1(fcent, f1):
+    """log10 Pr at which f1 takes the value +1 or -1: x = lp + c with x = n / 1.14 (f1 = 1) or
+    x = -n / 0.86 (f1 = -1)."""
+    lf = math.log10(fcent)
+    c, n = -0.4 - 0.67 * lf, 0.75 - 1.27 * lf
+    x = n / 1.14 if f1 > 0 else -n / 0.86
+    return x - c
+
+
+@pytest.mark.parametrize("a", [0.2, 0.43, 0.8])
+@pytest.mark.parametrize("f1", [1, -1])
+def test_troe_half_power_at_unit_f1(a, f1):
+    """At f1 = +-1 the broadening is F = Fcent^(1/2): pins 0.4, 0.67, 0.75, 1.27 and 0.14 together
+    (Fcent = a with T3 -> 0, T1 -> inf)."""
+    pr = 10.0 ** _troe_lp_for_f1(a, f1)
+    q, kc = _falloff_q("troe", pr, troe=(a, 1e-30, 1e30))
+    assert q == pytest.approx(kc * pr / (1 + pr) * math.sqrt(a), rel=1e-12)
+
+
+@pytest.mark.parametrize("lp", [2.0, -3.0, 0.5])
+def test_troe_limit_exponent_at_zero_n(lp):
+    """With Fcent = 10^(0.75/1.27), n = 0 and f1 = -1/0.14 for every Pr, so
+    F = Fcent^(1/(1 + (1/0.14)^2)) exactly: pins 0.14 (and the ratio 0.75 / 1.27)."""
+    a = 10.0 ** (0.75 / 1.27)
+    pr = 10.0 ** lp
+    q, kc = _falloff_q("troe", pr, troe=(a, 1e-30, 1e30))
+    F = a ** (1.0 / (1.0 + (1.0 / 0.14) ** 2))
+    assert q == pytest.approx(kc * pr / (1 + pr) * F, rel=1e-12)
+
+
+@pytest.mark.parametrize("lp", [250.0, -250.0])
+def test_troe_large_pr_limit(lp):
+    """|log10 Pr| -> inf: f1 -> -1/0.14, F -> Fcent^(1/(1 + (1/0.14)^2)) for any Fcent.  At
+    |log10 Pr| = 250 the distance of f1 from its limit is |n / (0.14 x)| < 4%, so log10 F is
+    within 10% of the limit (a wrong 0.14 moves it by 40% or more)."""
+    a = 0.43
+    pr = 10.0 ** lp
+    q, kc = _falloff_q("troe", pr, troe=(a, 1e-30, 1e30))
+    logF = math.log10(q / (kc * pr / (1 + pr)))
+    limit = math.log10(a) / (1.0 + (1.0 / 0.14) ** 2)
+    assert logF == pytest.approx(limit, rel=0.1)
+
+
+@pytest.mark.parametrize("T2,T", [(1000.0, 1500.0), (4000.0, 800.0), (250.0, 2200.0)])
+def test_troe_t2_term(T2, T):
+    """T3 -> 0, T1 -> inf: Fcent = a + exp(-T2/T); at f1 = 0 (log10 Pr = -c) F = Fcent."""
+    a = 0.3
+    fcent = a + math.exp(-T2 / T)
+    lf = math.log10(fcent)
+    pr = 10.0 ** (0.4 + 0.67 * lf)
+    q, kc = _falloff_q("troe", pr, T=T, troe=(a, 1e-30, 1e30, T2))
+    assert q == pytest.approx(kc * pr / (1 + pr) * fcent, rel=1e-12)
+
+
+@pytest.mark.parametrize("pr,X", [(10.0, 0.5), (0.1, 0.5), (100.0, 0.2), (1e-2, 0.2), (1e3, 0.1)])
+@pytest.mark.parametrize("sri", [(0.6, 300.0, 900.0, 1.3, 0.2), (0.45, 1200.0, 250.0)])
+def test_sri_exponent_closed_form(pr, X, sri):
+    """SRI (Kee, Coltrin & Glarborg, eq. 9.101): X = 1/(1 + log10(Pr)^2) is 1/2 at Pr = 10 and
+    0.1, 1/5 at Pr = 100 and 0.01, 1/10 at Pr = 1000; F = d (a e^(-b/T) + e^(-T/c))^X T^e
+    (d = 1, e = 0 for three parameters)."""
+    T = 1400.0
+    q, kc = _falloff_q("sri", pr, T=T, sri=sri)
+    a, b, c = sri[:3]
+    d, e = (sri[3], sri[4]) if len(sri) == 5 else (1.0, 0.0)
+    F = d * (a * math.exp(-b / T) + math.exp(-T / c)) ** X * T ** e
+    assert q == pytest.approx(kc * pr / (1 + pr) * F, rel=1e-12)
+
+
 def test_lindemann_limits():
     A, A0 = 2.0e12, 5.0e17
     r = _irrev({"H": 1.0, "O2": 1.0}, {"HO2": 1.0}, (A, 0.0, 0.0), rtype="lindemann", low=(A0, 0.0, 0.0))
@@ -303,3 +388,35 @@ def test_planted_mistakes_fail(monkeypatch):
     monkeypatch.setattr(og, "_smh", bad_smh)
     with pytest.raises(AssertionError):
         test_smh_derivative_is_enthalpy("H2O", 450.0)
+
+
+def _falloff_pins():
+    for a in (0.2, 0.43, 0.8):
+        for f1 in (1, -1):
+            test_troe_half_power_at_unit_f1(a, f1)
+    for lp in (2.0, -3.0, 0.5):
+        test_troe_limit_exponent_at_zero_n(lp)
+    for lp in (250.0, -250.0):
+        test_troe_large_pr_limit(lp)
+    for T2, T in ((1000.0, 1500.0), (4000.0, 800.0), (250.0, 2200.0)):
+        test_troe_t2_term(T2, T)
+    for pr, X in ((10.0, 0.5), (0.1, 0.5), (100.0, 0.2), (1e-2, 0.2), (1e3, 0.1)):
+        for sri in ((0.6, 300.0, 900.0, 1.3, 0.2), (0.45, 1200.0, 250.0)):
+            test_sri_exponent_closed_form(pr, X, sri)
+
+
+@pytest.mark.parametrize("plant", [
+    ("TROE_N1", -1.25),                     # n = 0.75 - 1.25 log10 Fcent
+    ("TROE_D", 0.11),                       # 0.11 in place of 0.14
+    ("TROE_C1", -0.76),                     # c = -0.4 - 0.76 log10 Fcent
+    ("TROE_N0", 0.7),                       # n = 0.70 - 1.27 log10 Fcent
+    ("troe_t2_term", lambda T2, T: np.exp(T2 / T)),              # exp(+T2/T)
+    ("sri_exponent", lambda lp: 1.0 / (1.0 + 2.0 * lp ** 2)),     # X = 1/(1 + 2 lp^2)
+])
+def test_planted_falloff_mistakes_fail(monkeypatch, plant):
+    """Each mistake planted in the oracle's falloff broadening makes a falloff pin fail, and the
+    unmodified oracle passes them all."""
+    _falloff_pins()
+    monkeypatch.setattr(og, *plant)
+    with pytest.raises(AssertionError):
+        _falloff_pins()
