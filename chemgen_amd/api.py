@@ -109,8 +109,9 @@ def _as_points(x, ng, dev):
 
 
 def getrates(P, T, Y, mech: Mechanism):
-    """One-shot convenience: load ``mech`` (cached per mechanism) and evaluate."""
-    key = (mech.name, structure_hash(mech))
+    """One-shot convenience: load ``mech`` (cached per mechanism structure and coefficient
+    values, so a mechanism with perturbed rates never reuses another's blob) and evaluate."""
+    key = cache_key(mech)
     g = _CACHE.get(key)
     if g is None:
         g = _CACHE[key] = GetRates(mech)
@@ -118,6 +119,12 @@ def getrates(P, T, Y, mech: Mechanism):
 
 
 _CACHE = {}
+
+
+def cache_key(mech: Mechanism):
+    """(name, structure hash, hash of the gpu_coef blob): one loaded mechanism per key."""
+    from .codegen import coef_hash
+    return (mech.name, structure_hash(mech), coef_hash(build_coef(mech)[0]))
 
 
 # ---- generated-module support (generate_chemistry.py --mode getrates --language hip) ----

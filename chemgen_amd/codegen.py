@@ -417,7 +417,7 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
             L.append(f"    const double base = cf[{o}] * cg_exp2(-cf[{o + 1}] * invT) + cg_exp2(-t * cf[{o + 2}]);")
             # k_inf F = d 2^(log2 k_inf + X log2 base [+ e log2 T]): one exp2 for all three
             l2F = "X * cg_log2(base)"
-            if len(r.sri) == 5 and r.sri[4] != 0.0:
+            if len(r.sri) == 5:   # compiled for e = 0 too: e is a coefficient, not structure
                 l2F = f"__builtin_fma(cf[{o + 4}], lg2T, {l2F})"
             L.append(f"    const double kf = cf[{o + 3}] * cg_exp2(lki + {l2F}) * cg_div(pr, 1.0 + pr);")
         else:
@@ -903,7 +903,7 @@ def _table_kind(r: Reaction) -> int:
     kind = _TABLE_KIND[r.rtype]
     if r.rtype == "troe" and len(r.troe) == 4:
         kind = 4
-    if r.rtype == "sri" and len(r.sri) == 5 and r.sri[4] != 0.0:
+    if r.rtype == "sri" and len(r.sri) == 5:   # e read from the blob, also when it is 0
         kind = 6
     return kind
 

@@ -256,13 +256,15 @@ __global__ void compact_k(const int* __restrict__ flag, int64_t ng, int* __restr
   if (m < ng && flag[m]) idx[atomicAdd(count, 1)] = m;
 }
 
-// gather P, T, Y of the listed points into packed arrays (ldo = capacity)
+// gather P, T, Y of the listed points into packed arrays (ldo >= cap); a device count above
+// cap is clamped to cap
 __global__ void gather_k(const double* __restrict__ P, const double* __restrict__ T,
                          const double* __restrict__ Y, int64_t ld, const int64_t* __restrict__ idx,
-                         const int* __restrict__ count, int ns, double* __restrict__ Po,
+                         const int* __restrict__ count, int64_t cap, int ns, double* __restrict__ Po,
                          double* __restrict__ To, double* __restrict__ Yo, int64_t ldo) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= *count) return;
+  const int64_t n = *count;
+  if (q >= (n < cap ? n : cap)) return;
   const int64_t m = idx[q];
   Po[q] = P[m];
   To[q] = T[m];
@@ -270,9 +272,11 @@ __global__ void gather_k(const double* __restrict__ P, const double* __restrict_
 }
 
 __global__ void scatter_k(const double* __restrict__ Wi, int64_t ldi, const int64_t* __restrict__ idx,
-                          const int* __restrict__ count, int ns, double* __restrict__ W, int64_t ld) {
+                          const int* __restrict__ count, int64_t cap, int ns, double* __restrict__ W,
+                          int64_t ld) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= *count) return;
+  const int64_t n = *count;
+  if (q >= (n < cap ? n : cap)) return;
   const int64_t m = idx[q];
   for (int k = 0; k < ns; ++k) W[(int64_t)k * ld + m] = Wi[(int64_t)k * ldi + q];
 }
@@ -727,7 +731,7 @@ extern "C" int cg_mlp_gather(const double* P, const double* T, const double* Y, 
   if (cap < 0 || ns <= 0 || ldo < cap || (cap > 0 && (!P || !T || !Y || !idx || !count || !Po || !To || !Yo)))
     return cg_internal_fail(CG_EINVAL, "cg_mlp_gather: bad argument");
   if (cap == 0) return CG_OK;
-  gather_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(P, T, Y, ld, idx, count, ns, Po, To, Yo, ldo);
+  gather_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(P, T, Y, ld, idx, count, cap, ns, Po, To, Yo, ldo);
   CG_LAUNCH_CHECK("cg_mlp_gather launch");
   return CG_OK;
 }
@@ -737,7 +741,7 @@ extern "C" int cg_mlp_scatter(const double* Wi, int64_t ldi, const int64_t* idx,
   if (cap < 0 || ns <= 0 || ldi < cap || (cap > 0 && (!Wi || !idx || !count || !W)))
     return cg_internal_fail(CG_EINVAL, "cg_mlp_scatter: bad argument");
   if (cap == 0) return CG_OK;
-  scatter_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(Wi, ldi, idx, count, ns, W, ld);
+  scatter_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(Wi, ldi, idx, count, cap, ns, W, ld);
   CG_LAUNCH_CHECK("cg_mlp_scatter launch");
   return CG_OK;
 }

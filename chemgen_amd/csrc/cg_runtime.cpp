@@ -2,8 +2,12 @@
 //
 // A mechanism handle owns: the hipModule loaded from the generated code object, the
 // kernels it exports (cg_getrates_kernel; cg_getrates_tiles_kernel, a persistent walk of a
-// tile list; cg_getrates_tile_kernel, one workgroup per tile of that list), the device
-// copy of the gpu_coef blob, and the tile list of the last prepared ragged batch.
+// tile list; cg_getrates_tile_kernel, one workgroup per tile of that list) and the device
+// copy of the gpu_coef blob.  A ragged batch (cg_batch) owns its tile list: a device copy and
+// the pinned host copy it is uploaded from, both alive until cg_batch_free, so a graph
+// captured over a launch of it stays valid for as long as the batch does.  The batches the
+// handle makes itself (cg_batched_prepare, cg_getrates_batched) are kept by the handle (see
+// there); none is freed while a launch may still read it.
 #include "chemgen_amd.h"
 
 #include <hip/hip_runtime.h>
@@ -51,6 +55,18 @@ uint64_t coef_fnv(const double* c, int64_t n) {
 
 }  // namespace
 
+struct cg_batch {
+  cg_mech* mech = nullptr;
+  void* dev = nullptr;       // device: tile starts (int64) then tile lengths (int32)
+  void* host = nullptr;      // pinned host copy the upload reads (kept until the batch is freed)
+  int64_t* tile_start_d = nullptr;
+  int32_t* tile_len_d = nullptr;
+  int64_t n_tiles = 0;
+  int64_t n_points = 0;      // offsets[n_patches]: a launch needs ld >= n_points
+  std::vector<int64_t> offsets;  // host copy (lets one-shot calls reuse an identical batch)
+  hipEvent_t done = nullptr;     // one-shot batches: recorded after their launch
+};
+
 struct cg_mech {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
@@ -60,12 +76,14 @@ struct cg_mech {
   Meta meta{};
   int device = 0;
   int n_cu = 0;
-  // ragged-batch tile list (device), capacity in tiles
-  int64_t* tile_start_d = nullptr;
-  int32_t* tile_len_d = nullptr;
-  int64_t tile_cap = 0;
-  int64_t n_tiles = 0;
-  int64_t n_points = 0;  // offsets[n_patches] of the prepared batch: the launch needs ld >= it
+  // cg_batched_prepare: the current prepared batch, and every earlier one (a graph captured
+  // over a launch of it may still be replayed), freed by cg_mech_free
+  cg_batch* prepared = nullptr;
+  std::vector<cg_batch*> kept;
+  // cg_getrates_batched: the batch of the last call (reused while the offsets repeat) and
+  // earlier ones whose launch may not have finished (freed once their event has completed)
+  cg_batch* oneshot = nullptr;
+  std::vector<cg_batch*> retired;
 };
 
 extern "C" const char* cg_last_error(void) { return g_err.c_str(); }
@@ -155,10 +173,16 @@ extern "C" int cg_mech_info(cg_mech_t h, int32_t* ns, int32_t* nr, int32_t* bloc
   return CG_OK;
 }
 
+static void batch_destroy(cg_batch* b);
+
 extern "C" int cg_mech_free(cg_mech_t h) {
   if (!h) return CG_OK;
-  if (h->tile_start_d) (void)hipFree(h->tile_start_d);
-  if (h->tile_len_d) (void)hipFree(h->tile_len_d);
+  // the caller guarantees that no launch on this handle is pending: hipFree waits for the
+  // device anyway
+  for (cg_batch* b : h->kept) batch_destroy(b);
+  for (cg_batch* b : h->retired) batch_destroy(b);
+  batch_destroy(h->prepared);
+  batch_destroy(h->oneshot);
   if (h->coef_d) (void)hipFree(h->coef_d);
   if (h->mod) (void)hipModuleUnload(h->mod);
   delete h;
@@ -180,47 +204,139 @@ extern "C" int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_
   return CG_OK;
 }
 
-extern "C" int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n_patches) {
-  if (!h || !offsets || n_patches < 0) return fail(CG_EINVAL, "cg_batched_prepare: bad argument");
-  if (offsets[0] != 0) return fail(CG_EINVAL, "cg_batched_prepare: offsets[0] must be 0");
-  // the device tile list may still be read by an earlier batched launch
-  if (h->n_tiles > 0) CG_HIP(hipDeviceSynchronize());
-  const int64_t B = h->meta.ppb;
-  std::vector<int64_t> start;
-  std::vector<int32_t> len;
+// ---- ragged batches ----
+
+extern "C" int64_t cg_batch_tiles(const int64_t* offsets, int32_t n_patches, int32_t tile,
+                                  int64_t* tile_start, int32_t* tile_len, int64_t cap) {
+  if (!offsets || n_patches < 0 || tile <= 0 || cap < 0 || (cap > 0 && (!tile_start || !tile_len)))
+    return fail(CG_EINVAL, "cg_batch_tiles: bad argument");
+  if (offsets[0] != 0) return fail(CG_EINVAL, "cg_batch_tiles: offsets[0] must be 0");
+  int64_t nt = 0;
   for (int32_t b = 0; b < n_patches; ++b) {
     const int64_t a = offsets[b], e = offsets[b + 1];
-    if (e < a) return fail(CG_EINVAL, "cg_batched_prepare: offsets must be non-decreasing");
-    // tiles aligned to multiples of B in the packed arrays (a short head tile first), so that
-    // every full tile reads and writes whole cache lines as the plain launch does
-    int64_t s = a;
-    while (s < e) {
-      int64_t t = (s / B + 1) * B;
+    if (e < a) return fail(CG_EINVAL, "cg_batch_tiles: offsets must be non-decreasing");
+    // tiles aligned to multiples of `tile` in the packed arrays (a short head tile first), so
+    // that every full tile reads and writes whole cache lines as the plain launch does
+    for (int64_t s = a; s < e;) {
+      int64_t t = (s / tile + 1) * tile;
       if (t > e) t = e;
-      start.push_back(s);
-      len.push_back((int32_t)(t - s));
+      if (nt < cap) {
+        tile_start[nt] = s;
+        tile_len[nt] = (int32_t)(t - s);
+      }
+      ++nt;
       s = t;
     }
   }
-  const int64_t nt = (int64_t)start.size();
-  if (nt > h->tile_cap) {
-    if (h->tile_start_d) (void)hipFree(h->tile_start_d);
-    if (h->tile_len_d) (void)hipFree(h->tile_len_d);
-    h->tile_start_d = nullptr;
-    h->tile_len_d = nullptr;
-    h->tile_cap = 0;
-    int64_t cap = nt + nt / 4 + 64;
-    if (hipMalloc(&h->tile_start_d, sizeof(int64_t) * cap) != hipSuccess ||
-        hipMalloc(&h->tile_len_d, sizeof(int32_t) * cap) != hipSuccess)
-      return fail(CG_ENOMEM, "cg_batched_prepare: tile list allocation failed");
-    h->tile_cap = cap;
-  }
+  return nt;
+}
+
+static void batch_destroy(cg_batch* b) {
+  if (!b) return;
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->dev) (void)hipFree(b->dev);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+}
+
+// Tile list of a batch in pinned host memory and on the device; the upload is enqueued on
+// `stream` (sync = false) or completed before returning (sync = true).
+static int batch_make(cg_mech* h, const int64_t* offsets, int32_t n_patches, hipStream_t stream,
+                      bool sync, cg_batch** out) {
+  *out = nullptr;
+  const int64_t nt = cg_batch_tiles(offsets, n_patches, h->meta.ppb, nullptr, nullptr, 0);
+  if (nt < 0) return (int)nt;
+  cg_batch* b = new cg_batch();
+  b->mech = h;
+  b->n_tiles = nt;
+  b->n_points = offsets[n_patches];
+  b->offsets.assign(offsets, offsets + n_patches + 1);
   if (nt > 0) {
-    CG_HIP(hipMemcpy(h->tile_start_d, start.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice));
-    CG_HIP(hipMemcpy(h->tile_len_d, len.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
+    const size_t bytes = (size_t)nt * (sizeof(int64_t) + sizeof(int32_t));
+    if (hipHostMalloc(&b->host, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      batch_destroy(b);
+      return fail(CG_ENOMEM, "ragged batch: pinned host allocation of %zu bytes failed", bytes);
+    }
+    if (hipMalloc(&b->dev, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      batch_destroy(b);
+      return fail(CG_ENOMEM, "ragged batch: device allocation of %zu bytes failed", bytes);
+    }
+    int64_t* hs = (int64_t*)b->host;
+    int32_t* hl = (int32_t*)(hs + nt);
+    (void)cg_batch_tiles(offsets, n_patches, h->meta.ppb, hs, hl, nt);
+    b->tile_start_d = (int64_t*)b->dev;
+    b->tile_len_d = (int32_t*)(b->tile_start_d + nt);
+    hipError_t e = sync ? hipMemcpy(b->dev, b->host, bytes, hipMemcpyHostToDevice)
+                        : hipMemcpyAsync(b->dev, b->host, bytes, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) {
+      batch_destroy(b);
+      return fail(CG_EHIP, "ragged batch: tile list upload failed: %s", hipGetErrorString(e));
+    }
   }
-  h->n_tiles = nt;
-  h->n_points = offsets[n_patches];
+  *out = b;
+  return CG_OK;
+}
+
+static int batch_launch(cg_batch* b, const double* P, const double* T, const double* Y, double* W,
+                        int64_t ld, int32_t grid, hipStream_t stream, const char* who) {
+  cg_mech* h = b->mech;
+  if (b->n_tiles == 0) return CG_OK;
+  if (!P || !T || !Y || !W) return fail(CG_EINVAL, "%s: null array", who);
+  if (ld < b->n_points)
+    return fail(CG_EINVAL, "%s: ld = %lld < %lld points of the batch", who, (long long)ld,
+                (long long)b->n_points);
+  // grid > 0: that many persistent workgroups; 0: 8 per CU; < 0: one workgroup per tile
+  int64_t g = grid > 0 ? grid : (grid < 0 ? b->n_tiles : (int64_t)h->n_cu * 8);
+  if (g > b->n_tiles) g = b->n_tiles;
+  if (g > 0x7fffffffLL) g = 0x7fffffffLL;
+  int64_t nt = b->n_tiles;
+  void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d,
+                  (void*)&b->tile_start_d, (void*)&b->tile_len_d, (void*)&nt, (void*)&ld};
+  // a grid that covers every tile runs the loop-free kernel (same tiles, one each)
+  hipFunction_t fn = (g == nt && h->fn_tile1) ? h->fn_tile1 : h->fn_tiles;
+  CG_HIP(hipModuleLaunchKernel(fn, (unsigned)g, 1, 1, (unsigned)h->meta.block, 1, 1, 0, stream, args,
+                               nullptr));
+  return CG_OK;
+}
+
+extern "C" int cg_batch_create(cg_mech_t h, const int64_t* offsets, int32_t n_patches, void* stream,
+                               cg_batch_t* out) {
+  if (!out) return fail(CG_EINVAL, "cg_batch_create: null output");
+  *out = nullptr;
+  if (!h || !offsets || n_patches < 0) return fail(CG_EINVAL, "cg_batch_create: bad argument");
+  return batch_make(h, offsets, n_patches, (hipStream_t)stream, false, out);
+}
+
+extern "C" int cg_batch_info(cg_batch_t b, int64_t* n_tiles, int64_t* n_points) {
+  if (!b) return fail(CG_EINVAL, "cg_batch_info: null batch");
+  if (n_tiles) *n_tiles = b->n_tiles;
+  if (n_points) *n_points = b->n_points;
+  return CG_OK;
+}
+
+extern "C" int cg_getrates_batch(cg_batch_t b, const double* P, const double* T, const double* Y,
+                                 double* W, int64_t ld, int32_t grid, void* stream) {
+  if (!b) return fail(CG_EINVAL, "cg_getrates_batch: null batch");
+  return batch_launch(b, P, T, Y, W, ld, grid, (hipStream_t)stream, "cg_getrates_batch");
+}
+
+extern "C" int cg_batch_free(cg_batch_t b) {
+  batch_destroy(b);
+  return CG_OK;
+}
+
+extern "C" int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n_patches) {
+  if (!h || !offsets || n_patches < 0) return fail(CG_EINVAL, "cg_batched_prepare: bad argument");
+  // a new batch every time; the previous one stays alive (a graph captured over a launch of
+  // it may still be replayed) until cg_mech_free.  The upload completes before returning, so
+  // a following launch on any stream sees it.
+  cg_batch* b = nullptr;
+  int rc = batch_make(h, offsets, n_patches, nullptr, true, &b);
+  if (rc != CG_OK) return rc;
+  if (h->prepared) h->kept.push_back(h->prepared);
+  h->prepared = b;
   return CG_OK;
 }
 
@@ -228,22 +344,9 @@ extern "C" int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const 
                                             const double* Y, double* W, int64_t ld, int32_t grid,
                                             void* stream) {
   if (!h) return fail(CG_EINVAL, "cg_getrates_batched_prepared: null handle");
-  if (h->n_tiles == 0) return CG_OK;
-  if (!P || !T || !Y || !W || ld < h->n_points)
-    return fail(CG_EINVAL, "cg_getrates_batched_prepared: bad array or ld < %lld points",
-                (long long)h->n_points);
-  // grid > 0: that many persistent workgroups; 0: 8 per CU; < 0: one workgroup per tile
-  int64_t g = grid > 0 ? grid : (grid < 0 ? h->n_tiles : (int64_t)h->n_cu * 8);
-  if (g > h->n_tiles) g = h->n_tiles;
-  if (g > 0x7fffffffLL) g = 0x7fffffffLL;
-  int64_t nt = h->n_tiles;
-  void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d,
-                  (void*)&h->tile_start_d, (void*)&h->tile_len_d, (void*)&nt, (void*)&ld};
-  // a grid that covers every tile runs the loop-free kernel (same tiles, one each)
-  hipFunction_t fn = (g == nt && h->fn_tile1) ? h->fn_tile1 : h->fn_tiles;
-  CG_HIP(hipModuleLaunchKernel(fn, (unsigned)g, 1, 1, (unsigned)h->meta.block, 1, 1, 0,
-                               (hipStream_t)stream, args, nullptr));
-  return CG_OK;
+  if (!h->prepared) return fail(CG_EINVAL, "cg_getrates_batched_prepared: no batch prepared");
+  return batch_launch(h->prepared, P, T, Y, W, ld, grid, (hipStream_t)stream,
+                      "cg_getrates_batched_prepared");
 }
 
 extern "C" int cg_getrates_batched(cg_mech_t h, const double* P, const double* T, const double* Y,
@@ -251,7 +354,48 @@ extern "C" int cg_getrates_batched(cg_mech_t h, const double* P, const double* T
                                    int32_t grid, void* stream) {
   if (!h || !offsets || n_patches < 0) return fail(CG_EINVAL, "cg_getrates_batched: bad argument");
   if (offsets[n_patches] > ld) return fail(CG_EINVAL, "cg_getrates_batched: offsets exceed ld");
-  int rc = cg_batched_prepare(h, offsets, n_patches);
+  hipStream_t st = (hipStream_t)stream;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CG_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(CG_EINVAL, "cg_getrates_batched: stream is capturing; capture cg_getrates_batch over a "
+                           "cg_batch_create'd batch instead");
+  // free the retired one-shot batches whose launches have finished
+  for (size_t i = 0; i < h->retired.size();) {
+    cg_batch* r = h->retired[i];
+    hipError_t q = hipEventQuery(r->done);
+    if (q == hipSuccess) {
+      batch_destroy(r);
+      h->retired[i] = h->retired.back();
+      h->retired.pop_back();
+    } else {
+      if (q != hipErrorNotReady) (void)hipGetLastError();
+      ++i;
+    }
+  }
+  cg_batch* b = h->oneshot;
+  const bool same = b && (int64_t)b->offsets.size() == (int64_t)n_patches + 1 &&
+                    std::memcmp(b->offsets.data(), offsets, sizeof(int64_t) * (n_patches + 1)) == 0;
+  if (!same) {
+    cg_batch* nb = nullptr;
+    int rc = batch_make(h, offsets, n_patches, st, false, &nb);
+    if (rc != CG_OK) return rc;
+    if (hipEventCreateWithFlags(&nb->done, hipEventDisableTiming) != hipSuccess) {
+      batch_destroy(nb);
+      return fail(CG_EHIP, "cg_getrates_batched: hipEventCreate failed");
+    }
+    if (b) {
+      if (b->done && hipEventQuery(b->done) == hipSuccess) {
+        batch_destroy(b);
+      } else {
+        (void)hipGetLastError();
+        h->retired.push_back(b);
+      }
+    }
+    h->oneshot = b = nb;
+  }
+  int rc = batch_launch(b, P, T, Y, W, ld, grid, st, "cg_getrates_batched");
   if (rc != CG_OK) return rc;
-  return cg_getrates_batched_prepared(h, P, T, Y, W, ld, grid, stream);
+  CG_HIP(hipEventRecord(b->done, st));
+  return CG_OK;
 }

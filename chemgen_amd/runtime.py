@@ -52,8 +52,15 @@ def lib():
             L.cg_getrates_batched.argtypes = [c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_i32, c_i32, c_p]
             L.cg_batched_prepare.argtypes = [c_p, c_p, c_i32]
             L.cg_getrates_batched_prepared.argtypes = [c_p, c_p, c_p, c_p, c_p, c_i64, c_i32, c_p]
+            L.cg_batch_create.argtypes = [c_p, c_p, c_i32, c_p, ctypes.POINTER(c_p)]
+            L.cg_batch_info.argtypes = [c_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]
+            L.cg_getrates_batch.argtypes = [c_p, c_p, c_p, c_p, c_p, c_i64, c_i32, c_p]
+            L.cg_batch_free.argtypes = [c_p]
+            L.cg_batch_tiles.argtypes = [c_p, c_i32, c_i32, c_p, c_p, c_i64]
+            L.cg_batch_tiles.restype = c_i64
             for f in ("cg_mech_load", "cg_mech_set_coef", "cg_mech_info", "cg_mech_free", "cg_getrates",
-                      "cg_getrates_batched", "cg_batched_prepare", "cg_getrates_batched_prepared"):
+                      "cg_getrates_batched", "cg_batched_prepare", "cg_getrates_batched_prepared",
+                      "cg_batch_create", "cg_batch_info", "cg_getrates_batch", "cg_batch_free"):
                 getattr(L, f).restype = ctypes.c_int
             _lib = L
     return _lib
@@ -161,12 +168,24 @@ class DeviceMechanism:
     def batched_prepare(self, offsets) -> None:
         off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
         check(lib().cg_batched_prepare(self._h, off.ctypes.data, len(off) - 1), "cg_batched_prepare")
+        self._prepared_points = int(off[-1])
 
     def getrates_batched_prepared_(self, P, T, Y, wdot, grid: int = 0, stream=None) -> None:
+        ng = getattr(self, "_prepared_points", None)
+        if ng is None:
+            raise ChemgenError("getrates_batched_prepared_: no batch prepared")
+        _check_dev(P, T, Y, wdot, self.n_species, ng)
         ld = _ld(Y)
+        if _ld(wdot) != ld:
+            raise ChemgenError("Y and wdot must share the leading dimension")
         check(lib().cg_getrates_batched_prepared(self._h, P.data_ptr(), T.data_ptr(), Y.data_ptr(),
                                                  wdot.data_ptr(), ld, grid, _stream_ptr(stream)),
               "cg_getrates_batched_prepared")
+
+    def batch(self, offsets, stream=None) -> "Batch":
+        """A ragged-batch object (cg_batch_create): its tile list lives until ``close()``, so
+        graphs captured over its launches stay valid that long."""
+        return Batch(self, offsets, stream)
 
     def close(self) -> None:
         if self._h:
@@ -178,6 +197,54 @@ class DeviceMechanism:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """A ragged batch of patches on one mechanism (owns its device tile list)."""
+
+    def __init__(self, dm: DeviceMechanism, offsets, stream=None):
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        self.dm = dm
+        self.n_points = int(off[-1])
+        self._b = ctypes.c_void_p()
+        check(lib().cg_batch_create(dm.handle, off.ctypes.data, len(off) - 1, _stream_ptr(stream),
+                                    ctypes.byref(self._b)), "cg_batch_create")
+        nt, npt = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().cg_batch_info(self._b, ctypes.byref(nt), ctypes.byref(npt)), "cg_batch_info")
+        self.n_tiles = nt.value
+
+    def getrates_(self, P, T, Y, wdot, grid: int = 0, stream=None) -> None:
+        _check_dev(P, T, Y, wdot, self.dm.n_species, self.n_points)
+        ld = _ld(Y)
+        if _ld(wdot) != ld:
+            raise ChemgenError("Y and wdot must share the leading dimension")
+        check(lib().cg_getrates_batch(self._b, P.data_ptr(), T.data_ptr(), Y.data_ptr(), wdot.data_ptr(),
+                                      ld, grid, _stream_ptr(stream)), "cg_getrates_batch")
+
+    def close(self) -> None:
+        if self._b:
+            lib().cg_batch_free(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_tiles(offsets, tile: int):
+    """(tile starts, tile lengths) of a ragged batch for ``tile``-point workgroups, as the
+    library builds them (cg_batch_tiles; host only)."""
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+    n = lib().cg_batch_tiles(off.ctypes.data, len(off) - 1, tile, None, None, 0)
+    if n < 0:
+        check(int(n), "cg_batch_tiles")
+    st = np.zeros(max(n, 1), dtype=np.int64)
+    ln = np.zeros(max(n, 1), dtype=np.int32)
+    n2 = lib().cg_batch_tiles(off.ctypes.data, len(off) - 1, tile, st.ctypes.data, ln.ctypes.data, n)
+    assert n2 == n
+    return st[:n], ln[:n]
 
 
 def _check_dev(P, T, Y, wdot, ns, ng):

@@ -68,21 +68,48 @@ int cg_mech_free(cg_mech_t h);
 int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_t ng, int64_t ld,
                 const double* Y, double* wdot, void* stream);
 
-/* Ragged batch in one launch.  Patch b covers points [offsets[b], offsets[b+1]) of the
- * packed arrays (offsets is a HOST array of n_patches+1 entries, offsets[0] = 0,
- * non-decreasing).  The tile list is built on the host and uploaded into a buffer the
- * handle owns (grown on demand), then one kernel launch covers it; the launch is
- * capturable into a hipGraph once the buffer is large enough.  `grid` = number of
- * persistent workgroups walking the tile list (0 = 8 per CU); negative, or at least the
- * number of tiles, = one workgroup per tile (a loop-free kernel over the same list).
- * Reference: no counterpart (the reference launches one kernel per array,
- * hip_v1_cpp.j2:339-372). */
+/* ---- ragged batches (no counterpart in the reference, which launches one kernel per array,
+ * hip_v1_cpp.j2:339-372) ----
+ * Patch b covers points [offsets[b], offsets[b+1]) of the packed arrays (offsets is a HOST
+ * array of n_patches+1 entries, offsets[0] = 0, non-decreasing).  The host splits the patches
+ * into tiles of at most the kernel's points per workgroup (never straddling a patch, aligned
+ * to tile boundaries of the packed arrays) and one launch walks the tile list.  `grid` =
+ * number of persistent workgroups walking the list (0 = 8 per CU); negative, or at least the
+ * number of tiles, = one workgroup per tile (a loop-free kernel over the same list). */
+
+typedef struct cg_batch* cg_batch_t;
+
+/* A batch object: builds the tile list of `offsets` for mechanism h and enqueues its upload
+ * on `stream` (launches on other streams must be ordered after it by the caller).  The batch
+ * owns its tile list (device memory and the pinned host copy the upload reads) until
+ * cg_batch_free, so a graph captured over cg_getrates_batch launches of it may be replayed
+ * for as long as the batch lives.  Create it outside stream capture. */
+int cg_batch_create(cg_mech_t h, const int64_t* offsets, int32_t n_patches, void* stream,
+                    cg_batch_t* out);
+int cg_batch_info(cg_batch_t b, int64_t* n_tiles, int64_t* n_points);
+/* One launch over the batch's tiles (capturable into a hipGraph); ld >= its point count. */
+int cg_getrates_batch(cg_batch_t b, const double* P, const double* T, const double* Y,
+                      double* wdot, int64_t ld, int32_t grid, void* stream);
+/* Frees the tile list; no launch or graph that uses the batch may still be pending. */
+int cg_batch_free(cg_batch_t b);
+
+/* The tile list itself, on the host (no GPU call): writes up to `cap` tiles of at most `tile`
+ * points and returns the number of tiles (a negative CG_E* code on bad input). */
+int64_t cg_batch_tiles(const int64_t* offsets, int32_t n_patches, int32_t tile,
+                       int64_t* tile_start, int32_t* tile_len, int64_t cap);
+
+/* One-shot form: builds (or reuses, when the offsets equal the previous call's) the handle's
+ * batch and launches it.  Earlier one-shot tile lists are freed only once the launch that read
+ * them has completed (an event per list), never by waiting.  Refused on a capturing stream:
+ * capture cg_getrates_batch over a batch object instead. */
 int cg_getrates_batched(cg_mech_t h, const double* P, const double* T, const double* Y,
                         double* wdot, int64_t ld, const int64_t* offsets, int32_t n_patches,
                         int32_t grid, void* stream);
 
-/* Prepare the tile list of a ragged batch without launching (so that a following
- * cg_getrates_batched_prepared can be captured into a hipGraph). */
+/* Handle-held form: cg_batched_prepare makes a new batch (upload complete on return) that the
+ * following cg_getrates_batched_prepared calls launch (capturable).  Every earlier prepared
+ * batch stays allocated until cg_mech_free, so graphs captured over earlier prepares remain
+ * valid after a re-prepare (cost: 12 bytes per tile per prepare). */
 int cg_batched_prepare(cg_mech_t h, const int64_t* offsets, int32_t n_patches);
 int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const double* T,
                                  const double* Y, double* wdot, int64_t ld, int32_t grid,

@@ -1,0 +1,39 @@
+"""What a compiled code object depends on: the mechanism's structure, not its coefficient values.
+
+One code object serves every coefficient set of its structure (cg_mech_set_coef, the
+generated module's blob), so nothing value-dependent may be compiled into the code unless
+the structure hash records it.  SRI's T^e factor is compiled whenever the SRI record has five
+parameters (e = 0 included), so e = 0 and e = 0.2 give the same hash and the same source.
+The one-shot API caches loaded mechanisms per (name, structure, blob).
+"""
+from chemgen_amd.api import cache_key
+from chemgen_amd.codegen import Variant, build_coef, emit_hip, structure_hash
+from chemgen_amd.kernels import load_bundled
+from chemgen_amd.mechanism import Mechanism, Reaction
+
+
+def _sri_mech(e, A=2.0e12):
+    base = load_bundled("h2_burke")
+    sp = [s for s in base.species if s.name in ("H", "O2", "HO2", "N2")]
+    r = Reaction("", {"H": 1.0, "O2": 1.0}, {"HO2": 1.0}, "sri", True, (A, 0.5, 300.0),
+                 low=(5.0e17, -1.0, 0.0), sri=(0.6, 300.0, 900.0, 1.3, e))
+    return Mechanism("sri_t", ["H", "O", "N"], sp, [r])
+
+
+def test_sri_e_is_a_coefficient():
+    m0, m1 = _sri_mech(0.0), _sri_mech(0.2)
+    assert structure_hash(m0) == structure_hash(m1)
+    for v in (Variant(), Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True),
+              Variant(64, 2, "lean", "gh", "horner", False, 4, 2, 0),
+              Variant(64, 1, "lean", "gh", "horner", False, 4, 2, 4)):
+        s0, s1 = emit_hip(m0, v)[0], emit_hip(m1, v)[0]
+        assert s0 == s1, v.tag
+    c0, c1 = build_coef(m0)[0], build_coef(m1)[0]
+    assert len(c0) == len(c1) and (c0 != c1).sum() == 1
+
+
+def test_cache_key_separates_coefficient_values():
+    a, b = _sri_mech(0.2), _sri_mech(0.2, A=3.0e12)
+    assert structure_hash(a) == structure_hash(b)
+    assert cache_key(a) != cache_key(b)
+    assert cache_key(a) == cache_key(_sri_mech(0.2))
