@@ -279,26 +279,32 @@ static int batch_make(cg_mech* h, const int64_t* offsets, int32_t n_patches, hip
   return CG_OK;
 }
 
-static int batch_launch(cg_batch* b, const double* P, const double* T, const double* Y, double* W,
+// One launch of the mechanism's tile kernel over nt tiles of a device tile list.
+static int tiles_launch(cg_mech* h, const int64_t* tile_start, const int32_t* tile_len, int64_t nt,
+                        int64_t n_points, const double* P, const double* T, const double* Y, double* W,
                         int64_t ld, int32_t grid, hipStream_t stream, const char* who) {
-  cg_mech* h = b->mech;
-  if (b->n_tiles == 0) return CG_OK;
+  if (nt == 0) return CG_OK;
   if (!P || !T || !Y || !W) return fail(CG_EINVAL, "%s: null array", who);
-  if (ld < b->n_points)
+  if (ld < n_points)
     return fail(CG_EINVAL, "%s: ld = %lld < %lld points of the batch", who, (long long)ld,
-                (long long)b->n_points);
+                (long long)n_points);
   // grid > 0: that many persistent workgroups; 0: 8 per CU; < 0: one workgroup per tile
-  int64_t g = grid > 0 ? grid : (grid < 0 ? b->n_tiles : (int64_t)h->n_cu * 8);
-  if (g > b->n_tiles) g = b->n_tiles;
+  int64_t g = grid > 0 ? grid : (grid < 0 ? nt : (int64_t)h->n_cu * 8);
+  if (g > nt) g = nt;
   if (g > 0x7fffffffLL) g = 0x7fffffffLL;
-  int64_t nt = b->n_tiles;
   void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d,
-                  (void*)&b->tile_start_d, (void*)&b->tile_len_d, (void*)&nt, (void*)&ld};
+                  (void*)&tile_start, (void*)&tile_len, (void*)&nt, (void*)&ld};
   // a grid that covers every tile runs the loop-free kernel (same tiles, one each)
   hipFunction_t fn = (g == nt && h->fn_tile1) ? h->fn_tile1 : h->fn_tiles;
   CG_HIP(hipModuleLaunchKernel(fn, (unsigned)g, 1, 1, (unsigned)h->meta.block, 1, 1, 0, stream, args,
                                nullptr));
   return CG_OK;
+}
+
+static int batch_launch(cg_batch* b, const double* P, const double* T, const double* Y, double* W,
+                        int64_t ld, int32_t grid, hipStream_t stream, const char* who) {
+  return tiles_launch(b->mech, b->tile_start_d, b->tile_len_d, b->n_tiles, b->n_points, P, T, Y, W, ld, grid,
+                      stream, who);
 }
 
 extern "C" int cg_batch_create(cg_mech_t h, const int64_t* offsets, int32_t n_patches, void* stream,
@@ -398,4 +404,36 @@ extern "C" int cg_getrates_batched(cg_mech_t h, const double* P, const double* T
   if (rc != CG_OK) return rc;
   CG_HIP(hipEventRecord(b->done, st));
   return CG_OK;
+}
+
+extern "C" int cg_internal_noalloc_tiles(const int64_t* offsets, int32_t n_patches, int32_t tile, int64_t nt,
+                                         void* stream, int64_t** start, int32_t** len);
+
+extern "C" int cg_getrates_batched_noalloc(cg_mech_t h, const double* P, const double* T, const double* Y,
+                                           double* W, int64_t ld, const int64_t* offsets, int32_t n_patches,
+                                           int32_t grid, void* stream) {
+  if (!h || !offsets || n_patches < 0) return fail(CG_EINVAL, "cg_getrates_batched_noalloc: bad argument");
+  if (n_patches > CG_NOALLOC_MAX_PATCHES)
+    return fail(CG_ERANGE, "cg_getrates_batched_noalloc: %d patches, at most %d", n_patches,
+                CG_NOALLOC_MAX_PATCHES);
+  // offsets checked and tiles counted on the host (count only: nothing is written or allocated)
+  const int64_t nt = cg_batch_tiles(offsets, n_patches, h->meta.ppb, nullptr, nullptr, 0);
+  if (nt < 0) return (int)nt;
+  if (nt > CG_NOALLOC_MAX_TILES)
+    return fail(CG_ERANGE, "cg_getrates_batched_noalloc: %lld tiles, at most %d (%d points per tile)",
+                (long long)nt, CG_NOALLOC_MAX_TILES, h->meta.ppb);
+  if (offsets[n_patches] > ld) return fail(CG_EINVAL, "cg_getrates_batched_noalloc: offsets exceed ld");
+  if (nt == 0) return CG_OK;
+  if (!P || !T || !Y || !W) return fail(CG_EINVAL, "cg_getrates_batched_noalloc: null array");
+  int dev = -1;
+  CG_HIP(hipGetDevice(&dev));
+  if (dev != h->device)
+    return fail(CG_EINVAL, "cg_getrates_batched_noalloc: current device %d, mechanism loaded on %d", dev,
+                h->device);
+  int64_t* ts = nullptr;
+  int32_t* tl = nullptr;
+  int rc = cg_internal_noalloc_tiles(offsets, n_patches, h->meta.ppb, nt, stream, &ts, &tl);
+  if (rc != CG_OK) return rc;
+  return tiles_launch(h, ts, tl, nt, offsets[n_patches], P, T, Y, W, ld, grid, (hipStream_t)stream,
+                      "cg_getrates_batched_noalloc");
 }

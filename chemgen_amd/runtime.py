@@ -58,9 +58,11 @@ def lib():
             L.cg_batch_free.argtypes = [c_p]
             L.cg_batch_tiles.argtypes = [c_p, c_i32, c_i32, c_p, c_p, c_i64]
             L.cg_batch_tiles.restype = c_i64
+            L.cg_getrates_batched_noalloc.argtypes = [c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_i32, c_i32, c_p]
             for f in ("cg_mech_load", "cg_mech_set_coef", "cg_mech_info", "cg_mech_free", "cg_getrates",
                       "cg_getrates_batched", "cg_batched_prepare", "cg_getrates_batched_prepared",
-                      "cg_batch_create", "cg_batch_info", "cg_getrates_batch", "cg_batch_free"):
+                      "cg_batch_create", "cg_batch_info", "cg_getrates_batch", "cg_batch_free",
+                      "cg_getrates_batched_noalloc"):
                 getattr(L, f).restype = ctypes.c_int
             _lib = L
     return _lib

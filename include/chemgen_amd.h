@@ -115,6 +115,21 @@ int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const double* T,
                                  const double* Y, double* wdot, int64_t ld, int32_t grid,
                                  void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * cg_getrates_batched without any allocation.  The tile list is built on the device into a
+ * fixed array of the library (CG_NOALLOC_MAX_TILES tiles, 6 MiB per GPU, held until the
+ * process ends) from the offsets, which travel by value in a kernel's arguments; at most
+ * CG_NOALLOC_MAX_PATCHES patches and CG_NOALLOC_MAX_TILES tiles (tiles of the kernel's
+ * points per workgroup), larger batches are refused with CG_ERANGE.  Results are identical
+ * to cg_getrates_batched's.  Capturable.  Calls that overlap on one GPU (other streams, a
+ * graph replayed concurrently) share the array and race: serialise them. */
+#define CG_NOALLOC_MAX_PATCHES 256
+#define CG_NOALLOC_MAX_TILES (1 << 19)
+int cg_getrates_batched_noalloc(cg_mech_t h, const double* P, const double* T, const double* Y,
+                                double* wdot, int64_t ld, const int64_t* offsets, int32_t n_patches,
+                                int32_t grid, void* stream);
+
 /* ---- learned surrogate (no counterpart in the reference: chemgen/chemistry_nn.py:15-85 is a
  * non-functional prototype).  BF16 values are passed as void* (16-bit, device memory). ---- */
 
