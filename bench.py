@@ -12,6 +12,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads: one per core, packed (set before any OpenMP runtime loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 
 WARM_S = 1.0   # untimed warm-up time before each timed region, seconds
@@ -59,26 +62,92 @@ def measured_fp64_peak(iters=20000):
     return best / 1e12
 
 
-def cpu_baseline(mech_name, mech, target_s=10.0, sample_ng=1 << 14):
-    """C/OpenMP oracle on the host cores over a bounded sample of the same states."""
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, CPUs this process may use, CPU model,
+    and the OpenMP settings."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def _avx512():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            flags = next((l for l in fh if l.startswith("flags")), "")
+        return all(f in flags.split() for f in ("avx512f", "avx512dq", "avx512vl", "avx512bw"))
+    except OSError:
+        return False
+
+
+def _median_rate(fn, npts, runs=5, min_s=1.0):
+    """Median over ``runs`` timed runs of points/s; each run repeats ``fn`` for at least
+    ``min_s`` seconds."""
     import numpy as np
+    fn()
+    rates = []
+    for _ in range(runs):
+        reps, t0 = 0, time.perf_counter()
+        while reps < 1 or time.perf_counter() - t0 < min_s:
+            fn()
+            reps += 1
+        rates.append(npts * reps / (time.perf_counter() - t0))
+    return float(np.median(rates)), rates
+
+
+def cpu_baseline(mech_name, mech, P, T, Y, target_s=15.0, sample_ng=1 << 14, host=None):
+    """The C/OpenMP restatement of the reference's getrates_i on the host cores, on the first
+    ``sample_ng`` of the GPU's own (seed 0) states: all OpenMP threads (OMP_PROC_BIND=close,
+    OMP_PLACES=cores) and one thread, median of 5 runs each, for the strict build (libm) and
+    the vectorised build (libmvec; AVX-512 where the CPU has it); ``value`` is the faster
+    all-thread figure.  Also the configs[0] Python-language reference: the NumPy oracle on the
+    H2 Burke mechanisms at ng = 4096, one thread."""
+    import numpy as np
+    from chemgen_amd.kernels import load_bundled
     from chemgen_amd.synth import make_states
     from oracle.build import COracle
-    co = COracle(mech_name)
-    P, T, Y = make_states(mech.n_species, sample_ng, seed=1)
-    co.getrates(P, T, Y)  # warm
-    t0 = time.perf_counter()
-    co.getrates(P, T, Y)
-    one = time.perf_counter() - t0
-    reps = max(1, min(5000, int(target_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        co.getrates(P, T, Y)
-    dt = time.perf_counter() - t0
-    return {"value": sample_ng * reps / dt, "unit": "points/s", "cores": co.threads,
+    from oracle.getrates_np import getrates as np_oracle
+    from oracle.metrics import parity_stats
+    n = sample_ng
+    Pc, Tc, Yc = P[:n].copy(), T[:n].copy(), np.ascontiguousarray(Y[:, :n])
+    ref, qf, qb = np_oracle(mech, Pc[:256], Tc[:256], Yc[:, :256], return_parts=True)
+    vec = "vec512" if _avx512() else "vec"
+    run_s = target_s / 5.0 / 4.0  # 4 timed configurations x 5 runs ~ target_s
+    builds = {}
+    for flavor in ("", vec):
+        co = COracle(mech_name, flavor)
+        par = parity_stats(mech, co.getrates(Pc[:256], Tc[:256], Yc[:, :256]), ref, qf, qb)
+        threads = co.threads
+        all_t, runs_all = _median_rate(lambda: co.getrates(Pc, Tc, Yc), n, 5, run_s)
+        co.set_threads(1)
+        one_t, runs_one = _median_rate(lambda: co.getrates(Pc[:4096], Tc[:4096], Yc[:, :4096]), 4096, 5, run_s)
+        co.set_threads(threads)
+        builds[flavor or "strict"] = {"threads": threads, "points_per_s": all_t, "runs": runs_all,
+                                      "one_thread_points_per_s": one_t, "one_thread_runs": runs_one,
+                                      "parity_max_gross": par["max_gross"]}
+    best = max(builds, key=lambda k: builds[k]["points_per_s"])
+    # configs[0]: the Python-language reference (NumPy oracle), one thread, ng = 4096
+    py = {}
+    for h2 in ("h2_burke2012", "h2_burke"):
+        m0 = load_bundled(h2)
+        P0, T0, Y0 = make_states(m0.n_species, 4096, seed=0)
+        rate, _ = _median_rate(lambda: np_oracle(m0, P0, T0, Y0), 4096, 5, 0.3)
+        py[h2] = {"points_per_s": rate, "ng": 4096}
+    return {"value": builds[best]["points_per_s"], "unit": "points/s", "cores": builds[best]["threads"],
             "kind": "port",
-            "sample": f"{reps} x {sample_ng} points of {mech_name} (seed 1), C/OpenMP restatement of "
-                      f"the reference's getrates_i, {dt:.1f} s"}
+            "sample": f"median of 5 runs of >= {run_s:.2f} s, each over the first {n} of the benchmark's "
+                      f"seed-0 states of {mech_name}; C/OpenMP restatement of the reference's getrates_i "
+                      f"({best} build)",
+            "build": best, "builds": builds, "host": host or host_info(),
+            "python_reference_ng4096": py}
 
 
 def bench_batched(steps, warmup):
@@ -229,18 +298,18 @@ def bench_h2_miniapp(steps, warmup, ngg=1 << 21, ng_cpu=1 << 18):
     nchk = 4096
     ref, qf, qb = np_oracle(mech, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
     par = parity_stats(mech, Wd[:, :nchk].cpu().numpy(), ref, qf, qb)
-    # one host core, C restatement of getrates_i
-    co = COracle(name)
-    threads0 = co.threads
-    co.set_threads(1)
+    # one host core, C restatement of getrates_i (vectorised build), on the full mechanism and
+    # on its 17 elementary reactions only (what the reference's CPU mini-app computes, see
+    # oracle.build.h2_burke_nofalloff)
     Pc, Tc, Yc = P[:ng_cpu].copy(), T[:ng_cpu].copy(), np.ascontiguousarray(Y[:, :ng_cpu])
-    co.getrates(Pc, Tc, Yc)
-    reps, t0 = 0, time.perf_counter()
-    while reps < 3 or time.perf_counter() - t0 < 2.0:
-        co.getrates(Pc, Tc, Yc)
-        reps += 1
-    t_cpu = (time.perf_counter() - t0) / reps
-    co.set_threads(threads0)
+    vec = "vec512" if _avx512() else "vec"
+    one_core = {}
+    for nm in (name, "h2_burke_nofalloff"):
+        co = COracle(nm, vec)
+        threads0 = co.threads
+        co.set_threads(1)
+        one_core[nm], _ = _median_rate(lambda: co.getrates(Pc, Tc, Yc), ng_cpu, 5, 0.5)
+        co.set_threads(threads0)
     w = work_per_point(mech)
     achieved_bw = w["bytes"] * ngg / t
     traffic = None
@@ -257,8 +326,11 @@ def bench_h2_miniapp(steps, warmup, ngg=1 << 21, ng_cpu=1 << 18):
             "published_gfx90a_points_per_s": H2_PUBLISHED_GPU,
             "hbm_frac": achieved_bw / PEAK_HBM, "fp64_frac": w["flops"] * ngg / t / PEAK_FP64_VECTOR,
             "traffic": traffic,
-            "cpu_one_core_points_per_s": ng_cpu / t_cpu,
-            "cpu_one_core_sample": f"{reps} x {ng_cpu} points, C restatement of getrates_i, 1 OpenMP thread",
+            "cpu_one_core_points_per_s": one_core[name],
+            "cpu_one_core_elementary_only_points_per_s": one_core["h2_burke_nofalloff"],
+            "cpu_one_core_sample": f"median of 5 runs over {ng_cpu} points, C restatement of getrates_i "
+                                   f"({vec} build), 1 OpenMP thread; elementary_only: without the 4 "
+                                   f"third-body and 2 Troe reactions",
             "published_cpu_one_core_points_per_s": H2_PUBLISHED_CPU_CORE,
             "parity": {"sample_points": nchk, **par}}
 
@@ -327,7 +399,7 @@ def bench_surrogate(steps, warmup, ng=1 << 20):
     W = torch.empty_like(Y)
     flag, _ = sur.predict_(P, T, Y, W)
     err_sur = sur.error(W, ref)
-    info = sur.getrates_(P, T, Y, W, dm)
+    info = sur.fallback_info(sur.getrates_(P, T, Y, W, dm))
     err = sur.error(W, ref)
     s = torch.cuda.current_stream()
 
@@ -392,6 +464,7 @@ def main():
     ap.add_argument("--mode", choices=["getrates", "copy"], default="getrates",
                     help="copy: a plain 1 GiB device copy, the calibration for HBM byte counters")
     args = ap.parse_args()
+    host = host_info()   # before any OpenMP runtime binds this thread to its first place
     if args.gpus != 1:
         print("bench.py measures one GPU only: the getrates path is per-point independent and "
               "this build runs on a single MI355X; --gpus must be 1", file=sys.stderr)
@@ -444,7 +517,35 @@ def main():
     ref, qf, qb = np_oracle(mech, P[:nchk], T[:nchk], Y[:, :nchk], return_parts=True)
     par = parity_stats(mech, Wc, ref, qf, qb)
 
+    # the 40-digit arbiter on the first 64 points: raw and gross errors of the GPU and of the
+    # FP64 oracle against the exact value of the reference formula
+    from oracle.arbiter_mp import arbiter_stats, getrates_mp
+    na = 64
+    wm, G = getrates_mp(mech, P[:na], T[:na], Y[:, :na])
+    par["arbiter"] = {"sample_points": na, "gpu": arbiter_stats(Wc[:, :na], wm, G),
+                      "oracle": arbiter_stats(ref[:, :na], wm, G)}
+
     w = work_per_point(mech)
+    # the other sizes of the GRI-sized configuration: the first n points of the same resident
+    # arrays (leading dimension 2^20), timed the same way
+    sizes = {}
+    for n in (1 << 16, 1 << 18):
+        if n >= ng:
+            continue
+        a = (Pd[:n], Td[:n], Yd[:, :n], Wd[:, :n])
+        warm_up(lambda: dm.getrates_(*a, stream), args.warmup)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(args.steps, (args.steps * ng) // n)
+        e0.record(stream)
+        for _ in range(reps):
+            dm.getrates_(*a, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        tn = e0.elapsed_time(e1) / reps / 1e3
+        sizes[str(n)] = {"points_per_s": n / tn, "kernel_avg_ms": tn * 1e3,
+                         "roofline_frac": w["flops"] * n / tn / PEAK_FP64_VECTOR}
+    sizes[str(ng)] = {"points_per_s": ng / avg_launch_s, "kernel_avg_ms": avg_launch_s * 1e3,
+                      "roofline_frac": w["flops"] * ng / avg_launch_s / PEAK_FP64_VECTOR}
     achieved = w["flops"] * ng / avg_launch_s
     peak_meas = measured_fp64_peak()
     traffic = None
@@ -479,6 +580,7 @@ def main():
                      "kernel_avg_ms": avg_launch_s * 1e3,
                      "measured_peak": peak_meas, "frac_of_measured": achieved / 1e12 / peak_meas},
         "parity": {"sample_points": nchk, **par},
+        "sizes": sizes,
     }
     if not args.no_batched:
         del Pd, Td, Yd, Wd
@@ -503,7 +605,7 @@ def main():
         torch.cuda.empty_cache()
         out["qss"] = bench_qss(args.steps, args.warmup)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.mech, mech, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.mech, mech, P, T, Y, args.cpu_seconds, host=host)
     print(json.dumps(_finite(out)))
 
 

@@ -84,12 +84,15 @@ class Variant:
     live: bool = False       # lane kernel: reactions ordered to keep few species live, each
                              # species' state computed at its first use and its wdot stored after
                              # its last (register pressure of the lane-per-point kernel)
+    pair: bool = False       # straight-line kernels: the exp2 of two consecutive Arrhenius rates
+                             # (and of two species' EG) evaluated interleaved, so that each
+                             # wavefront has two independent FMA chains in flight
 
     @property
     def tag(self) -> str:
         if self.table:
             return f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
-        c = ("k" if self.baked else "") + ("o" if self.live else "")
+        c = ("k" if self.baked else "") + ("o" if self.live else "") + ("p" if self.pair else "")
         if self.split > 1:
             return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
                     f"{self.poly[0]}g{self.group}{c}")
@@ -335,11 +338,12 @@ def _plog_code(r: Reaction, off: int) -> List[str]:
 
 
 def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp",
-                   skip_eff=frozenset(), qss=None) -> List[str]:
+                   skip_eff=frozenset(), qss=None, kf_var: str = None) -> List[str]:
     """Straight-line code of one reaction: kf (and M), q and the wdot updates.  ``skip_eff``:
     species left out of the third-body sum.  ``qss`` = (QSS species -> unknown index, stage):
     stage 1 emits the reaction's terms of the QSS balance, stage 2 its rate with the solved
-    QSS concentrations x<i> (see chemgen_amd/qssa.py)."""
+    QSS concentrations x<i> (see chemgen_amd/qssa.py).  ``kf_var``: an elementary or
+    third-body reaction's 2^(log2 k) was computed before (``_paired_code``) into this variable."""
     L = [f"  {{  // R{j}: {_eq(r)}" + (f" [QSS stage {qss[1]}]" if qss else "")]
     if kr_mode == "gh":
         cvar = lambda k: f"(g{k} * h{k})"
@@ -362,8 +366,11 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
     tb_off = None
     if r.rtype in ("standard", "third_body"):
         sg = "-" if negative_a(r) else ""
-        L.append(f"    const double lk = {_lk(off, r.arrh[1], r.arrh[2])};")
-        L.append(f"    const double kf = {sg}cg_exp2_fin(lk);")
+        if kf_var is not None:
+            L.append(f"    const double kf = {sg}{kf_var};")
+        else:
+            L.append(f"    const double lk = {_lk(off, r.arrh[1], r.arrh[2])};")
+            L.append(f"    const double kf = {sg}cg_exp2_fin(lk);")
         if r.reversible and kr_mode != "gh":
             if kr_mode == "eg":
                 L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
@@ -456,6 +463,39 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
         L.append(f"    w{idx[s]} += {'q' if n == 1.0 else dlit(n) + ' * q'};")
     L.append("  }")
     return L
+
+
+def _pairable(r: Reaction, kr_mode: str) -> bool:
+    """An Arrhenius rate whose only transcendental is its 2^(log2 k)."""
+    return r.rtype in ("standard", "third_body") and kr_mode == "gh"
+
+
+def _reactions_code(js: List[int], mech: Mechanism, roffs: List[int], idx: Dict[str, int], kr_mode: str,
+                    pair: bool, indent: str = "") -> List[List[str]]:
+    """Code blocks of the reactions ``js`` in order.  With ``pair``, two consecutive Arrhenius
+    rates share one block whose two exp2 evaluations are interleaved (cg_exp2_fin2), then each
+    reaction's body follows as before; returns one block per reaction or pair."""
+    out: List[List[str]] = []
+    q = 0
+    while q < len(js):
+        j = js[q]
+        r = mech.reactions[j]
+        if pair and q + 1 < len(js) and _pairable(r, kr_mode) and _pairable(mech.reactions[js[q + 1]], kr_mode):
+            j2 = js[q + 1]
+            r2 = mech.reactions[j2]
+            L = [f"  {{  // R{j} and R{j2}: their rates' exp2 interleaved",
+                 "    double kfp0, kfp1;",
+                 f"    cg_exp2_fin2({_lk(roffs[j], r.arrh[1], r.arrh[2])}, "
+                 f"{_lk(roffs[j2], r2.arrh[1], r2.arrh[2])}, kfp0, kfp1);"]
+            L += _reaction_code(j, r, roffs[j], idx, kr_mode, kf_var="kfp0")
+            L += _reaction_code(j2, r2, roffs[j2], idx, kr_mode, kf_var="kfp1")
+            L.append("  }")
+            out.append([indent + ln for ln in L])
+            q += 2
+        else:
+            out.append([indent + ln for ln in _reaction_code(j, r, roffs[j], idx, kr_mode)])
+            q += 1
+    return out
 
 
 def _eq(r: Reaction) -> str:
@@ -836,11 +876,15 @@ def _emit_split(mech: Mechanism, variant: Variant):
     for v in range(W):
         lines: List[str] = []
         tv = set()
-        for q, j in enumerate(parts[v]):
-            if q > 0 and q % max(1, variant.group) == 0:
+        gsz = max(1, variant.group)
+        for g0 in range(0, len(parts[v]), gsz):
+            if g0 > 0:
                 lines.append("    CG_FENCE();")
+            for blk in _reactions_code(parts[v][g0:g0 + gsz], mech, roffs, idx, variant.kr_mode,
+                                       variant.pair, "  "):
+                lines.extend(blk)
+        for j in parts[v]:
             r = mech.reactions[j]
-            lines.extend("  " + ln for ln in _reaction_code(j, r, roffs[j], idx, variant.kr_mode))
             tv |= {idx[s] for s in r.reactants} | {idx[s] for s in r.products}
         codes.append("\n".join(lines))
         touched.append(sorted(tv))
@@ -861,6 +905,7 @@ def _emit_split(mech: Mechanism, variant: Variant):
         r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of, species_of=species_of,
         zero_of=species_of, touched=touched, reactions_code=codes, compact_lds=compact_lds,
+        pair=variant.pair,
         **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,

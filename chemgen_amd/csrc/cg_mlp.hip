@@ -249,39 +249,162 @@ __global__ void decode_k(const float* __restrict__ Zh, int64_t ng, int ns, const
   flag[m] = (r <= tol) ? 0 : 1;
 }
 
-// compaction of flagged points: idx[count++] = m
-__global__ void compact_k(const int* __restrict__ flag, int64_t ng, int* __restrict__ count,
-                          int64_t* __restrict__ idx) {
+// Compaction of the flagged points: idx[*count ...] = m for flag[m] != 0 and, with
+// audit_every > 0, m % audit_every != 0 (the audited points keep their FP64 values); nothing
+// while *skip is set.  A workgroup takes kCompactChunk consecutive points, numbers its flagged
+// ones with a prefix sum in LDS and reserves their slots with one atomic add, so the counter
+// sees one atomic per 4096 points (same-address atomics serialise in L2).
+constexpr int kCompactPer = 16;                       // points per thread
+constexpr int kCompactChunk = 256 * kCompactPer;      // points per workgroup
+
+__global__ void __launch_bounds__(256)
+compact_k(const int* __restrict__ flag, int64_t ng, int audit_every, int* __restrict__ count,
+          int64_t* __restrict__ idx, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ int scan[256];
+  __shared__ int base;
+  const int64_t b0 = (int64_t)blockIdx.x * kCompactChunk;
+  unsigned mask = 0;
+#pragma unroll
+  for (int i = 0; i < kCompactPer; ++i) {
+    const int64_t m = b0 + (int64_t)i * 256 + threadIdx.x;
+    const bool f = m < ng && flag[m] != 0 && (audit_every <= 0 || m % audit_every != 0);
+    mask |= f ? (1u << i) : 0u;
+  }
+  const int mine = __popc(mask);
+  scan[threadIdx.x] = mine;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {            // inclusive Hillis-Steele scan
+    const int v = (int)threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 255) base = scan[255] ? atomicAdd(count, scan[255]) : 0;
+  __syncthreads();
+  int q = base + scan[threadIdx.x] - mine;
+#pragma unroll
+  for (int i = 0; i < kCompactPer; ++i)
+    if (mask & (1u << i)) idx[q++] = b0 + (int64_t)i * 256 + threadIdx.x;
+}
+
+// The surrogate audit's verdict: one thread per audited point, off = some species with
+// |Ws - Wr| / (|Wr| + s_k) > err_tol (or not finite); the integer count is order-independent,
+// so the verdict is deterministic.  The last workgroup to finish (a ticket in acc[1]) sets
+// *n_bad and *reject = (*n_bad > max_bad * n), zeroes *count_clear (the compaction's counter,
+// when given) and clears acc for the next call.
+__global__ void __launch_bounds__(256)
+audit_k(const double* __restrict__ Ws, int64_t lds, const double* __restrict__ Wr, int64_t ldr, int64_t n,
+        int ns, const double* __restrict__ s, double err_tol, double max_bad, int* __restrict__ reject,
+        int* __restrict__ n_bad, unsigned* __restrict__ acc, int* __restrict__ count_clear) {
+  __shared__ int red;
+  if (threadIdx.x == 0) red = 0;
+  __syncthreads();
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < ng && flag[m]) idx[atomicAdd(count, 1)] = m;
+  if (m < n) {
+    bool off = false;
+    for (int k = 0; k < ns; ++k) {
+      const double wr = Wr[(int64_t)k * ldr + m];
+      const double q = fabs(Ws[(int64_t)k * lds + m] - wr) / (fabs(wr) + s[k]);
+      off |= !(q <= err_tol);
+    }
+    if (off) atomicAdd(&red, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (red) atomicAdd(&acc[0], (unsigned)red);
+    __threadfence();
+    if (atomicAdd(&acc[1], 1u) == gridDim.x - 1) {
+      __threadfence();
+      const unsigned bad = atomicAdd(&acc[0], 0u);
+      *n_bad = (int)bad;
+      *reject = (double)bad > max_bad * (double)n ? 1 : 0;
+      if (count_clear) *count_clear = 0;
+      acc[0] = 0u;
+      acc[1] = 0u;
+    }
+  }
+}
+
+// The fallback plan and the tile lengths of its two FP64 launches over [0, ng) in tiles of
+// `tile` points (tile t starts at t tile): a rejected audit, or more than dense_frac of the
+// points flagged, sends the whole batch to one launch (*full = ng, *pcount = 0); otherwise
+// *full = 0 and the *pcount = *count flagged points are recomputed packed.  Lengths:
+// clamp(*pcount - t tile, 0, tile) and clamp(*full - t tile, 0, tile); zero-length tiles do no
+// work.
+__global__ void fallback_plan_k(const int* __restrict__ count, const int* __restrict__ reject, int64_t ng,
+                                double dense_frac, int64_t nt, int tile, int* __restrict__ full,
+                                int* __restrict__ pcount, int* __restrict__ len_packed, int* __restrict__ len_full) {
+  const int c = *count;
+  const bool all = *reject != 0 || (double)c > dense_frac * (double)ng;
+  const int64_t np = all ? 0 : c, nf = all ? ng : 0;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    *full = (int)nf;
+    *pcount = (int)np;
+  }
+  if (t >= nt) return;
+  const int64_t a = t * tile;
+  int64_t lp = np - a, lf = nf - a;
+  lp = lp < 0 ? 0 : (lp > tile ? tile : lp);
+  lf = lf < 0 ? 0 : (lf > tile ? tile : lf);
+  len_packed[t] = (int)lp;
+  len_full[t] = (int)lf;
+}
+
+// A tile list for the first *count points of [0, cap): tile t = [t tile, min((t+1) tile, *count)),
+// length 0 beyond the count (the getrates tile kernels skip those)
+__global__ void count_tiles_k(const int* __restrict__ count, int64_t nt, int tile, int64_t* __restrict__ start,
+                              int* __restrict__ len) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int64_t a = t * tile, c = *count;
+  int64_t l = c - a;
+  l = l < 0 ? 0 : (l > tile ? tile : l);
+  start[t] = a;
+  len[t] = (int)l;
 }
 
 // gather P, T, Y of the listed points into packed arrays (ldo >= cap); a device count above
-// cap is clamped to cap
+// cap is clamped to cap.  A fixed grid strides over (row, point) pairs, row ns holding P and T,
+// so an empty list costs one small launch and a long one spreads over the whole GPU.
 __global__ void gather_k(const double* __restrict__ P, const double* __restrict__ T,
                          const double* __restrict__ Y, int64_t ld, const int64_t* __restrict__ idx,
                          const int* __restrict__ count, int64_t cap, int ns, double* __restrict__ Po,
                          double* __restrict__ To, double* __restrict__ Yo, int64_t ldo) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n = *count;
-  if (q >= (n < cap ? n : cap)) return;
-  const int64_t m = idx[q];
-  Po[q] = P[m];
-  To[q] = T[m];
-  for (int k = 0; k < ns; ++k) Yo[(int64_t)k * ldo + q] = Y[(int64_t)k * ld + m];
+  const int64_t c = *count;
+  const int64_t n = c < cap ? c : cap;
+  const int64_t total = n * (ns + 1);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / n, q = e - k * n;
+    const int64_t m = idx[q];
+    if (k < ns) {
+      Yo[k * ldo + q] = Y[k * ld + m];
+    } else {
+      Po[q] = P[m];
+      To[q] = T[m];
+    }
+  }
 }
 
 __global__ void scatter_k(const double* __restrict__ Wi, int64_t ldi, const int64_t* __restrict__ idx,
                           const int* __restrict__ count, int64_t cap, int ns, double* __restrict__ W,
                           int64_t ld) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n = *count;
-  if (q >= (n < cap ? n : cap)) return;
-  const int64_t m = idx[q];
-  for (int k = 0; k < ns; ++k) W[(int64_t)k * ld + m] = Wi[(int64_t)k * ldi + q];
+  const int64_t c = *count;
+  const int64_t n = c < cap ? c : cap;
+  const int64_t total = n * ns;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / n, q = e - k * n;
+    W[k * ld + idx[q]] = Wi[k * ldi + q];
+  }
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+// grid of the grid-stride kernels: one thread per element, at most 1024 workgroups of 256
+inline unsigned strided_grid(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
 
 
 // ---------------- fused inference ----------------
@@ -472,9 +595,12 @@ mlp_infer_k(const double* __restrict__ P, const double* __restrict__ T, const do
             int64_t ld, int64_t ng, int ns, const float* __restrict__ fs, const bf16x8* __restrict__ Wf,
             const float* __restrict__ bias, int kc0, int n_hidden, int nt_out, const double* __restrict__ s,
             const float* __restrict__ aek, int ne, double tol, double* __restrict__ W, int64_t ldw,
-            int* __restrict__ flag, double* __restrict__ resid) {
+            int* __restrict__ flag, double* __restrict__ resid, const int* __restrict__ skip) {
   using S = InferShape<H>;
   constexpr int PT = S::PT, NP = S::NP, LPP = S::LPP;
+  // a device-side verdict (the surrogate's audit) can cancel the launch: every workgroup reads
+  // it first, before any memory traffic, and leaves
+  if (skip && *skip) return;
   extern __shared__ __attribute__((aligned(16))) bf16x8 wlds[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t base = ((int64_t)blockIdx.x * S::NW + wave) * NP;
@@ -717,11 +843,49 @@ extern "C" int cg_mlp_decode(const float* Zh, int64_t ng, int32_t ns, const doub
   return CG_OK;
 }
 
-extern "C" int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t* count, int64_t* idx, void* stream) {
-  if (ng < 0 || (ng > 0 && (!flag || !count || !idx))) return cg_internal_fail(CG_EINVAL, "cg_mlp_compact: bad argument");
+extern "C" int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t audit_every, int32_t* count, int64_t* idx,
+                              const int32_t* skip, void* stream) {
+  if (ng < 0 || ng > 0x7fffffffLL || (ng > 0 && (!flag || !count || !idx)))
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_compact: bad argument");
   if (ng == 0) return CG_OK;
-  compact_k<<<nblk(ng, 256), 256, 0, (hipStream_t)stream>>>(flag, ng, count, idx);
+  compact_k<<<nblk(ng, kCompactChunk), 256, 0, (hipStream_t)stream>>>(flag, ng, audit_every, count, idx, skip);
   CG_LAUNCH_CHECK("cg_mlp_compact launch");
+  return CG_OK;
+}
+
+extern "C" int cg_mlp_audit(const double* Ws, int64_t lds, const double* Wr, int64_t ldr, int64_t n, int32_t ns,
+                            const double* s, double err_tol, double max_bad, int32_t* reject, int32_t* n_bad,
+                            uint32_t* acc, int32_t* count_clear, void* stream) {
+  if (n <= 0 || ns <= 0 || lds < n || ldr < n || !reject || !n_bad || !acc || !Ws || !Wr || !s)
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_audit: bad argument");
+  if (n > 0x7fffffffLL) return cg_internal_fail(CG_ERANGE, "cg_mlp_audit: n too large");
+  audit_k<<<nblk(n, 256), 256, 0, (hipStream_t)stream>>>(Ws, lds, Wr, ldr, n, ns, s, err_tol, max_bad, reject, n_bad,
+                                                         acc, count_clear);
+  CG_LAUNCH_CHECK("cg_mlp_audit launch");
+  return CG_OK;
+}
+
+extern "C" int cg_mlp_fallback_plan(const int32_t* count, const int32_t* reject, int64_t ng, double dense_frac,
+                                    int32_t tile, int32_t* full, int32_t* pcount, int32_t* len_packed,
+                                    int32_t* len_full, void* stream) {
+  if (ng <= 0 || ng > 0x7fffffffLL || tile <= 0 || !count || !reject || !full || !pcount || !len_packed ||
+      !len_full)
+    return cg_internal_fail(CG_EINVAL, "cg_mlp_fallback_plan: bad argument");
+  const int64_t nt = (ng + tile - 1) / tile;
+  fallback_plan_k<<<nblk(nt, 256), 256, 0, (hipStream_t)stream>>>(count, reject, ng, dense_frac, nt, tile, full,
+                                                                  pcount, len_packed, len_full);
+  CG_LAUNCH_CHECK("cg_mlp_fallback_plan launch");
+  return CG_OK;
+}
+
+extern "C" int cg_count_tiles(const int32_t* count, int64_t cap, int32_t tile, int64_t* tile_start,
+                              int32_t* tile_len, void* stream) {
+  if (cap < 0 || tile <= 0 || !count || (cap > 0 && (!tile_start || !tile_len)))
+    return cg_internal_fail(CG_EINVAL, "cg_count_tiles: bad argument");
+  const int64_t nt = (cap + tile - 1) / tile;
+  if (nt == 0) return CG_OK;
+  count_tiles_k<<<nblk(nt, 256), 256, 0, (hipStream_t)stream>>>(count, nt, tile, tile_start, tile_len);
+  CG_LAUNCH_CHECK("cg_count_tiles launch");
   return CG_OK;
 }
 
@@ -731,7 +895,8 @@ extern "C" int cg_mlp_gather(const double* P, const double* T, const double* Y, 
   if (cap < 0 || ns <= 0 || ldo < cap || (cap > 0 && (!P || !T || !Y || !idx || !count || !Po || !To || !Yo)))
     return cg_internal_fail(CG_EINVAL, "cg_mlp_gather: bad argument");
   if (cap == 0) return CG_OK;
-  gather_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(P, T, Y, ld, idx, count, cap, ns, Po, To, Yo, ldo);
+  gather_k<<<strided_grid(cap * (ns + 1)), 256, 0, (hipStream_t)stream>>>(P, T, Y, ld, idx, count, cap, ns, Po, To,
+                                                                          Yo, ldo);
   CG_LAUNCH_CHECK("cg_mlp_gather launch");
   return CG_OK;
 }
@@ -741,7 +906,7 @@ extern "C" int cg_mlp_scatter(const double* Wi, int64_t ldi, const int64_t* idx,
   if (cap < 0 || ns <= 0 || ldi < cap || (cap > 0 && (!Wi || !idx || !count || !W)))
     return cg_internal_fail(CG_EINVAL, "cg_mlp_scatter: bad argument");
   if (cap == 0) return CG_OK;
-  scatter_k<<<nblk(cap, 256), 256, 0, (hipStream_t)stream>>>(Wi, ldi, idx, count, cap, ns, W, ld);
+  scatter_k<<<strided_grid(cap * ns), 256, 0, (hipStream_t)stream>>>(Wi, ldi, idx, count, cap, ns, W, ld);
   CG_LAUNCH_CHECK("cg_mlp_scatter launch");
   return CG_OK;
 }
@@ -759,7 +924,7 @@ template <int H>
 int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng, int32_t ns,
                  const float* fscale, const void* Wf, const float* bias, int32_t kc0, int32_t n_hidden,
                  int32_t nt_out, const double* s, const float* aek, int32_t ne, double tol, double* W,
-                 int64_t ldw, int32_t* flag, double* resid, hipStream_t stream) {
+                 int64_t ldw, int32_t* flag, double* resid, const int32_t* skip, hipStream_t stream) {
   using S = InferShape<H>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -772,7 +937,7 @@ int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, 
   if (nb > 0x7fffffffLL) return cg_internal_fail(CG_ERANGE, "cg_mlp_infer: ng too large");
   mlp_infer_k<H><<<(unsigned)nb, 64 * S::NW, S::LDS_BYTES, stream>>>(
       P, T, Y, ld, ng, ns, fscale, (const bf16x8*)Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W, ldw,
-      flag, resid);
+      flag, resid, skip);
   CG_LAUNCH_CHECK("cg_mlp_infer launch");
   return CG_OK;
 }
@@ -781,7 +946,8 @@ int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, 
 extern "C" int cg_mlp_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng,
                             int32_t ns, const float* fscale, const void* Wf, const float* bias, int32_t hidden,
                             int32_t kin, int32_t n_hidden, const double* s, const float* aek, int32_t ne,
-                            double tol, double* W, int64_t ldw, int32_t* flag, double* resid, void* stream) {
+                            double tol, double* W, int64_t ldw, int32_t* flag, double* resid,
+                            const int32_t* skip, void* stream) {
   if (ng == 0) return CG_OK;
   if (ng < 0 || ld < ng || ldw < ng || ns <= 0 || n_hidden < 1 || ne < 0 || ne > 8)
     return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: bad argument");
@@ -792,7 +958,7 @@ extern "C" int cg_mlp_infer(const double* P, const double* T, const double* Y, i
     return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: input/output wider than the hidden layers");
   if (hidden == 256)
     return launch_infer<256>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
-                             ldw, flag, resid, (hipStream_t)stream);
+                             ldw, flag, resid, skip, (hipStream_t)stream);
   return launch_infer<512>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
-                           ldw, flag, resid, (hipStream_t)stream);
+                           ldw, flag, resid, skip, (hipStream_t)stream);
 }

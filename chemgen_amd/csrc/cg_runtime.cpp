@@ -406,6 +406,15 @@ extern "C" int cg_getrates_batched(cg_mech_t h, const double* P, const double* T
   return CG_OK;
 }
 
+extern "C" int cg_getrates_tiles(cg_mech_t h, const double* P, const double* T, const double* Y, double* W,
+                                 int64_t ld, const int64_t* tile_start, const int32_t* tile_len, int64_t n_tiles,
+                                 int64_t n_points, int32_t grid, void* stream) {
+  if (!h || n_tiles < 0 || n_points < 0 || (n_tiles > 0 && (!tile_start || !tile_len)))
+    return fail(CG_EINVAL, "cg_getrates_tiles: bad argument");
+  return tiles_launch(h, tile_start, tile_len, n_tiles, n_points, P, T, Y, W, ld, grid, (hipStream_t)stream,
+                      "cg_getrates_tiles");
+}
+
 extern "C" int cg_internal_noalloc_tiles(const int64_t* offsets, int32_t n_patches, int32_t tile, int64_t nt,
                                          void* stream, int64_t** start, int32_t** len);
 

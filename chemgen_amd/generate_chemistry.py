@@ -12,8 +12,10 @@
   <stem>.json, .inp  mechanism metadata and the mechanism as parsed
 ``--mode gpu_coef`` writes DIR/gpu_coef_cdna4.npy (+ .json): the coefficient blob the
 kernels read (the layout is documented in chemgen_amd/codegen.py); with ``--compat
-reference`` also DIR/coef_m_vN.f90 (N = --parallel-level, at most 3), the reference's
-Fortran coefficient module that its ``chemgen_m`` uses (chemgen_amd/legacy_coef.py).
+reference`` also DIR/coef_m_vN.f90 (N = --parallel-level), the reference's Fortran
+coefficient module that its ``chemgen_m`` uses (chemgen_amd/legacy_coef.py): species maps at
+levels 1-3 (MAX_SP a power of two at 3), dense rocBLAS matrices at 4, each reaction type padded
+to a multiple of --nreact-per-block with the reference's dummy reactions.
 
 Launch shape, the analogue of the reference's parallel levels (--parallel-level,
 --veclen, --nreact-per-block):
@@ -119,10 +121,12 @@ def generate_gpu_coefficients(args) -> None:
     if args.compat == "reference":
         # the reference's Fortran coefficient module, for chemgen_m over the legacy library
         from .legacy_coef import emit_coef_module
-        level = min(args.parallel_level, 3)
+        level = args.parallel_level
         path = os.path.join(args.output, f"coef_m_v{level}.f90")
         with open(path, "w") as fh:
-            fh.write(emit_coef_module(mech, level))
+            # the reference's CLI always pads each reaction type to a multiple of
+            # --nreact-per-block (generate_chemistry.py:56-70)
+            fh.write(emit_coef_module(mech, level, args.nreact_per_block))
         print(f"reference coefficient module written to {path}")
     print(f"gpu_coef written to {args.output}")
 
@@ -160,7 +164,7 @@ def getrates(P, T, Y):
 ''')
     if args.legacy_abi:
         from .legacy_abi import build_legacy_library
-        build_legacy_library(mech, os.path.abspath(stem + ".hsaco"), stem + "_legacy.so")
+        build_legacy_library(mech, os.path.abspath(stem + ".hsaco"), stem + "_legacy.so", args.parallel_level)
         print(f"reference C ABI library written to {stem}_legacy.so")
     print(f"Expressions and getrates function written to {args.output}")
 

@@ -57,7 +57,35 @@ def _params(present) -> List[str]:
     return p
 
 
-def emit_legacy_source(mech: Mechanism, code_object: str) -> str:
+def _params4(present) -> List[str]:
+    """copyConstantsToDevice of the rocBLAS level (chemgen/constants.py:327-420): dense
+    coefficient matrices instead of species maps."""
+    p = []
+    if "standard" in present:
+        p += ["const double* A_h", "const double* B_h", "const double* sk_coef_h", "const double* coef_r_h",
+              "const double* coef_p_h", "const double* wdot_coef_h"]
+    if "troe" in present:
+        p += ["const double* A_0_troe_h", "const double* B_0_troe_h", "const double* A_inf_troe_h",
+              "const double* B_inf_troe_h", "const double* sk_coef_troe_h", "const double* coef_r_troe_h",
+              "const double* coef_p_troe_h", "const double* eff_fac_troe_h", "const double* fcent_coef_troe_h",
+              "const double* wdot_coef_troe_h"]
+    if "third_body" in present:
+        p += ["const double* A_third_h", "const double* B_third_h", "const double* sk_coef_third_h",
+              "const double* coef_r_third_h", "const double* coef_p_third_h", "const double* eff_fac_third_h",
+              "const double* wdot_coef_third_h"]
+    p += ["const double* mw_h", "const double* smh_coef_h", "const double* T_mid_h"]
+    return p
+
+
+def emit_legacy_source(mech: Mechanism, code_object: str, level: int = 1) -> str:
+    """C++ of the compatibility library for the coefficient layout of parallel level
+    ``level``: levels 1-3 pass species maps with MAX_SP entries per side (rounded up to a power
+    of two at level 3), level 4 the dense matrices of the rocBLAS path with another argument
+    list.  Dummy reactions that pad a type to a multiple of the reactions per block follow its
+    real reactions in every array, so they need no handling here."""
+    from .legacy_coef import max_species
+    if level not in (1, 2, 3, 4):
+        raise ValueError("parallel level must be 1..4")
     for r in mech.reactions:
         if r.rtype not in _TYPES:
             raise ValueError(f"the reference's HIP ABI has no {r.rtype} reactions (load with compat='reference')")
@@ -65,8 +93,11 @@ def emit_legacy_source(mech: Mechanism, code_object: str) -> str:
     idx = {s.name: i for i, s in enumerate(mech.species)}
     present = [t for t in _TYPES if any(r.rtype == t for r in mech.reactions)]
     counters = {t: 0 for t in _TYPES}
-    max_sp = max(max(len(r.reactants), len(r.products)) for r in mech.reactions)
+    max_sp = max_species(mech, level)
+    ns_ = mech.n_species
     rows, effsp, checks = [], [], []
+    suffix = {0: "", 1: "_troe", 2: "_third"}
+    chk_lines = []
     for j, r in enumerate(mech.reactions):
         t = _TYPES.index(r.rtype)
         i = counters[r.rtype]
@@ -74,15 +105,20 @@ def emit_legacy_source(mech: Mechanism, code_object: str) -> str:
         t2 = int(r.rtype == "troe" and len(r.troe) == 4)
         rows.append(f"  {{{t}, {i}, {roffs[j]}, {t2}, {len(r.third_body)}, {len(effsp)}}}")
         effsp += [idx[s] for s in r.third_body]
-        for q, s in enumerate(r.reactants):
-            checks.append((t, f"{i * max_sp + q}", idx[s], "r"))
-        for q, s in enumerate(r.products):
-            checks.append((t, f"{i * max_sp + q}", idx[s], "p"))
-    suffix = {0: "", 1: "_troe", 2: "_third"}
-    chk_lines = []
-    for (t, pos, k, side) in checks:
-        chk_lines.append(f"  if (map_{side}{suffix[t]}_h[{pos}] != {k}) return 2;")
-    params = _params(present)
+        sfx = suffix[t]
+        if level < 4:
+            for q, s_ in enumerate(r.reactants):
+                chk_lines.append(f"  if (map_r{sfx}_h[{i * max_sp + q}] != {idx[s_]}) return 2;")
+            for q, s_ in enumerate(r.products):
+                chk_lines.append(f"  if (map_p{sfx}_h[{i * max_sp + q}] != {idx[s_]}) return 2;")
+        else:
+            # the dense rows of this reaction: its coefficients where its species are, and row
+            # sums that leave no room for another species
+            for side, arr in ((r.reactants, "coef_r"), (r.products, "coef_p")):
+                for s_, nu in side.items():
+                    chk_lines.append(f"  if ({arr}{sfx}_h[{i * ns_ + idx[s_]}] != {float(nu)!r}) return 2;")
+                chk_lines.append(f"  if (rowsum({arr}{sfx}_h + {i * ns_}) != {float(sum(side.values()))!r}) return 2;")
+    params = _params(present) if level < 4 else _params4(present)
     args = ", ".join(p.split()[-1] for p in params)
     h = structure_hash(mech)
     ns, nr = mech.n_species, mech.n_reactions
@@ -121,7 +157,8 @@ def emit_legacy_source(mech: Mechanism, code_object: str) -> str:
             "(r.type == 0 ? 0.0 : eff_fac_third_h[(long)r.idx * NS + kEffSp[r.eoff + q]])", "0.0")
     src = f'''// Generated by chemgen_amd: the reference's C ABI (copyConstantsToDevice,
 // getrates_hip_wrapper_v1..v4, allocate_intermediate_DeviceMemory) for mechanism
-// '{mech.name}' ({ns} species, {nr} reactions) over this build's generated getrates kernel.
+// '{mech.name}' ({ns} species, {nr} reactions) over this build's generated getrates kernel,
+// for the coefficient arrays of parallel level {level}{" (species maps with MAX_SP = %d)" % max_sp if level < 4 else " (dense rocBLAS matrices)"}.
 #include "chemgen_amd.h"
 
 #include <hip/hip_runtime.h>
@@ -158,6 +195,12 @@ const char* code_object() {{
 // The reference's arrays -> this build's gpu_coef blob (cf, NCOEF doubles).  Returns 0, or 1
 // for a NULL argument, 2 if the species maps do not match this mechanism, 3 for a Troe form
 // (with / without T2) or T_mid that differs from it.
+static double rowsum(const double* a) {{
+  double s = 0.0;
+  for (int k = 0; k < NS; ++k) s += std::fabs(a[k]);
+  return s;
+}}
+
 extern "C" int cg_legacy_convert({", ".join(params)}, double* cf) {{
   if (!cf || !mw_h || !smh_coef_h || !T_mid_h) return 1;
 {chr(10).join(chk_lines)}
@@ -230,9 +273,10 @@ extern "C" void allocate_intermediate_DeviceMemory(int ng) {{ (void)ng; }}
     return src
 
 
-def build_legacy_library(mech: Mechanism, code_object: str, out_path: str) -> str:
-    """Write and compile the compatibility library (links libchemgen_amd.so)."""
-    src = emit_legacy_source(mech, code_object)
+def build_legacy_library(mech: Mechanism, code_object: str, out_path: str, level: int = 1) -> str:
+    """Write and compile the compatibility library for parallel level ``level`` (links
+    libchemgen_amd.so)."""
+    src = emit_legacy_source(mech, code_object, level)
     src_path = os.path.splitext(out_path)[0] + ".cpp"
     with open(src_path, "w") as fh:
         fh.write(src)

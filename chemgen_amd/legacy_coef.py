@@ -20,9 +20,10 @@ reference's array names, types, sizes and values, so that no reference tooling i
   for the low then the high range of every species (:116-194).
 
 Numbers are printed as the reference prints them: ``%21.15E`` with a ``D`` exponent for
-reals, ``%.2E`` for T_mid, ``%.1f`` for stoichiometric coefficients.  The arrays are those of
-parallel level 1 (no dummy reactions: the compatibility library takes the mechanism's own
-reaction counts); the module name carries the level that ``chemgen_m`` is compiled for.
+reals, ``%.2E`` for T_mid, ``%.1f`` for stoichiometric coefficients.  Level 3 rounds MAX_SP up
+to a power of two; level 4 replaces the maps by the dense matrices of the reference's rocBLAS
+path; with reactions per block given, each type is padded with the reference's dummy reactions
+(write_constants.py:85-90).
 """
 from __future__ import annotations
 
@@ -39,16 +40,57 @@ def _d(x: float) -> str:
     return f"{x:21.15E}".replace("E", "D").strip()
 
 
-def _arrays(mech: Mechanism) -> List[Tuple[str, str, List[str]]]:
-    """(Fortran type, name, literal values) in the module's order."""
-    idx = {s.name: i for i, s in enumerate(mech.species)}
+class _Dummy:
+    """The reference's padding reaction (chemgen/chemistry.py:143-183): the first species with
+    coefficient 0 on both sides, A = 1, beta = Ea = 0; Troe low limit (1, 0, 0) and centre
+    (1, 1, 1e30); three-body efficiencies {first species: 0} (eff - 1)."""
+
+    def __init__(self, mech, rtype):
+        sp0 = mech.species[0].name
+        self.rtype = rtype
+        self.reactants = {sp0: 0.0}
+        self.products = {sp0: 0.0}
+        self.arrh = (1.0, 0.0, 0.0)
+        self.low = (1.0, 0.0, 0.0)
+        self.troe = (1.0, 1.0, 1e30)
+        self.third_body = {sp0: 0.0} if rtype == "third_body" else {}
+
+
+def max_species(mech: Mechanism, level: int) -> int:
+    """MAX_SP of the level's arrays: the most species on one side of a reaction, rounded up to a
+    power of two at level 3 (chemgen/chemistry.py:124-133, called with level > 2 by
+    write_constants.py:202,244,285)."""
+    m = max(max(len(r.reactants), len(r.products)) for r in mech.reactions)
+    return 2 ** math.ceil(math.log2(m)) if level > 2 else m
+
+
+def reactions_by_type(mech: Mechanism, rpb=None) -> Dict[str, list]:
+    """The reactions of each type the reference's HIP arrays hold, in mechanism order, each
+    type padded with dummy reactions to a multiple of ``rpb`` when it is given
+    (write_constants.py:85-90, repeated in every writer)."""
     by_type: Dict[str, list] = {t: [] for t in _SUFFIX}
     for r in mech.reactions:
         if r.rtype not in by_type:
             raise ValueError(f"the reference's coefficient module holds no {r.rtype} reactions")
         by_type[r.rtype].append(r)
+    if rpb:
+        for t, rs in by_type.items():
+            if rs and len(rs) % rpb:
+                rs += [_Dummy(mech, t) for _ in range(rpb - len(rs) % rpb)]
+    return by_type
+
+
+def _arrays(mech: Mechanism, level: int = 1, rpb=None) -> List[Tuple[str, str, List[str]]]:
+    """(Fortran type, name, literal values) in the module's order (write_constants.py:456-493)."""
+    if level not in (1, 2, 3, 4):
+        raise ValueError("parallel level must be 1..4")
+    if level > 1 and not rpb:
+        raise ValueError("parallel levels 2-4 need reactions per block (the reference asserts it)")
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    ns = mech.n_species
+    by_type = reactions_by_type(mech, rpb)
     present = [t for t in _SUFFIX if by_type[t]]
-    max_sp = max(max(len(r.reactants), len(r.products)) for r in mech.reactions)
+    max_sp = max_species(mech, level)
     out: List[Tuple[str, str, List[str]]] = []
     real, integer = "real(kind=8)", "integer"
 
@@ -69,24 +111,45 @@ def _arrays(mech: Mechanism) -> List[Tuple[str, str, List[str]]]:
             ab = [arrhenius(r.arrh) for r in rs]
             out.append((real, f"A{_SUFFIX[t]}_h", [a for a, _ in ab]))
             out.append((real, f"B{_SUFFIX[t]}_h", [b for _, bb in ab for b in bb]))
-    for t in present:
-        sk, mr, mp = [], [], []
-        for r in by_type[t]:
-            for side, m in ((r.reactants, mr), (r.products, mp)):
-                ks = [str(idx[s]) for s in side] + ["0"] * (max_sp - len(side))
-                sk += ks
-                m += ks
-        sfx = _SUFFIX[t]
-        out += [(integer, f"sk_map{sfx}_h", sk), (integer, f"map_r{sfx}_h", mr), (integer, f"map_p{sfx}_h", mp)]
-    for t in present:
-        sk, cr, cp = [], [], []
-        for r in by_type[t]:
-            sk += [f"{-n:.1f}" for n in r.reactants.values()] + ["0.0"] * (max_sp - len(r.reactants))
-            sk += [f"{n:.1f}" for n in r.products.values()] + ["0.0"] * (max_sp - len(r.products))
-            cr += [f"{n:.1f}" for n in r.reactants.values()] + ["0.0"] * (max_sp - len(r.reactants))
-            cp += [f"{n:.1f}" for n in r.products.values()] + ["0.0"] * (max_sp - len(r.products))
-        sfx = _SUFFIX[t]
-        out += [(real, f"sk_coef{sfx}_h", sk), (real, f"coef_r{sfx}_h", cr), (real, f"coef_p{sfx}_h", cp)]
+    if level < 4:
+        for t in present:
+            sk, mr, mp = [], [], []
+            for r in by_type[t]:
+                for side, m in ((r.reactants, mr), (r.products, mp)):
+                    ks = [str(idx[s]) for s in side] + ["0"] * (max_sp - len(side))
+                    sk += ks
+                    m += ks
+            sfx = _SUFFIX[t]
+            out += [(integer, f"sk_map{sfx}_h", sk), (integer, f"map_r{sfx}_h", mr), (integer, f"map_p{sfx}_h", mp)]
+        for t in present:
+            sk, cr, cp = [], [], []
+            for r in by_type[t]:
+                sk += [f"{-n:.1f}" for n in r.reactants.values()] + ["0.0"] * (max_sp - len(r.reactants))
+                sk += [f"{n:.1f}" for n in r.products.values()] + ["0.0"] * (max_sp - len(r.products))
+                cr += [f"{n:.1f}" for n in r.reactants.values()] + ["0.0"] * (max_sp - len(r.reactants))
+                cp += [f"{n:.1f}" for n in r.products.values()] + ["0.0"] * (max_sp - len(r.products))
+            sfx = _SUFFIX[t]
+            out += [(real, f"sk_coef{sfx}_h", sk), (real, f"coef_r{sfx}_h", cr), (real, f"coef_p{sfx}_h", cp)]
+    else:
+        # level 4: dense matrices for the rocBLAS products (write_constants.py:350-406):
+        # sk_coef/coef_r/coef_p [reaction][species], wdot_coef [species][reaction]
+        for t in present:
+            rs = by_type[t]
+            nr = len(rs)
+            sk, cr, cp, wd = [0.0] * (nr * ns), [0.0] * (nr * ns), [0.0] * (nr * ns), [0.0] * (nr * ns)
+            for q, r in enumerate(rs):
+                for side, sign in ((r.reactants, -1.0), (r.products, 1.0)):
+                    for sp, nu in side.items():
+                        sk[q * ns + idx[sp]] += sign * nu
+                        wd[q + idx[sp] * nr] += sign * nu
+                for sp, nu in r.reactants.items():
+                    cr[q * ns + idx[sp]] += nu
+                for sp, nu in r.products.items():
+                    cp[q * ns + idx[sp]] += nu
+            sfx = _SUFFIX[t]
+            f1 = lambda a: [f"{v:.1f}" for v in a]
+            out += [(real, f"sk_coef{sfx}_h", f1(sk)), (real, f"coef_r{sfx}_h", f1(cr)),
+                    (real, f"coef_p{sfx}_h", f1(cp)), (real, f"wdot_coef{sfx}_h", f1(wd))]
     for t in ("troe", "third_body"):
         if by_type[t]:
             vals = [_d(1.0 + r.third_body.get(s.name, 0.0)) for r in by_type[t] for s in mech.species]
@@ -109,12 +172,13 @@ def _arrays(mech: Mechanism) -> List[Tuple[str, str, List[str]]]:
     return out
 
 
-def emit_coef_module(mech: Mechanism, level: int = 1) -> str:
-    """Free-form Fortran source of ``module coef_m_v<level>`` (see the module docstring)."""
-    if level not in (1, 2, 3):
-        raise ValueError("coefficient module: levels 1-3 (level 4 passes rocBLAS matrices instead)")
+def emit_coef_module(mech: Mechanism, level: int = 1, rpb=None) -> str:
+    """Free-form Fortran source of ``module coef_m_v<level>`` (see the module docstring):
+    levels 1-3 with species maps (MAX_SP rounded to a power of two at level 3), level 4 with the
+    dense matrices of the rocBLAS path; each reaction type padded with the reference's dummy
+    reactions to a multiple of ``rpb`` when it is given (its CLI always gives it, default 8)."""
     lines = [f"module coef_m_v{level}", "  implicit none", ""]
-    for ftype, name, vals in _arrays(mech):
+    for ftype, name, vals in _arrays(mech, level, rpb):
         lines.append(f"  {ftype} :: {name}({len(vals)})=(/&")
         per = 4
         for i in range(0, len(vals), per):
@@ -124,3 +188,22 @@ def emit_coef_module(mech: Mechanism, level: int = 1) -> str:
         lines.append("  /)")
     lines += ["", f"end module coef_m_v{level}", ""]
     return "\n".join(lines)
+
+
+def parse_coef_module(text: str) -> Dict[str, "object"]:
+    """The arrays of a coefficient module as NumPy arrays (int32 for ``integer``, float64 for
+    ``real(kind=8)``), by name: what ``chemgen_m`` passes to copyConstantsToDevice."""
+    import re
+    import numpy as np
+    out = {}
+    for m in re.finditer(r"(integer|real\(kind=8\))\s*::\s*(\w+)\((\d+)\)\s*=\s*\(/(.*?)/\)", text, re.S):
+        ftype, name, n, body = m.group(1), m.group(2), int(m.group(3)), m.group(4)
+        toks = [t for t in re.split(r"[,\s&]+", body) if t]
+        if ftype == "integer":
+            arr = np.array([int(t) for t in toks], dtype=np.int32)
+        else:
+            arr = np.array([float(t.replace("D", "E").replace("d", "e")) for t in toks], dtype=np.float64)
+        if arr.size != n:
+            raise ValueError(f"{name}: {arr.size} values for size {n}")
+        out[name] = arr
+    return out

@@ -59,10 +59,11 @@ def lib():
             L.cg_batch_tiles.argtypes = [c_p, c_i32, c_i32, c_p, c_p, c_i64]
             L.cg_batch_tiles.restype = c_i64
             L.cg_getrates_batched_noalloc.argtypes = [c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_i32, c_i32, c_p]
+            L.cg_getrates_tiles.argtypes = [c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_i64, c_i64, c_i32, c_p]
             for f in ("cg_mech_load", "cg_mech_set_coef", "cg_mech_info", "cg_mech_free", "cg_getrates",
                       "cg_getrates_batched", "cg_batched_prepare", "cg_getrates_batched_prepared",
                       "cg_batch_create", "cg_batch_info", "cg_getrates_batch", "cg_batch_free",
-                      "cg_getrates_batched_noalloc"):
+                      "cg_getrates_batched_noalloc", "cg_getrates_tiles"):
                 getattr(L, f).restype = ctypes.c_int
             _lib = L
     return _lib
@@ -183,6 +184,18 @@ class DeviceMechanism:
         check(lib().cg_getrates_batched_prepared(self._h, P.data_ptr(), T.data_ptr(), Y.data_ptr(),
                                                  wdot.data_ptr(), ld, grid, _stream_ptr(stream)),
               "cg_getrates_batched_prepared")
+
+    def getrates_tiles_(self, P, T, Y, wdot, tile_start, tile_len, n_points: int, grid: int = -1,
+                        stream=None) -> None:
+        """One launch over a device tile list (int64 starts, int32 lengths; zero-length tiles are
+        skipped) of points in [0, n_points)."""
+        _check_dev(P, T, Y, wdot, self.n_species, n_points)
+        ld = _ld(Y)
+        if _ld(wdot) != ld:
+            raise ChemgenError("Y and wdot must share the leading dimension")
+        check(lib().cg_getrates_tiles(self._h, P.data_ptr(), T.data_ptr(), Y.data_ptr(), wdot.data_ptr(), ld,
+                                      tile_start.data_ptr(), tile_len.data_ptr(), int(tile_start.numel()),
+                                      n_points, grid, _stream_ptr(stream)), "cg_getrates_tiles")
 
     def batch(self, offsets, stream=None) -> "Batch":
         """A ragged-batch object (cg_batch_create): its tile list lives until ``close()``, so

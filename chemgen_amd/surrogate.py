@@ -48,11 +48,14 @@ def _bind():
         "cg_mlp_adam": [p, p, p, p, i64, f32, f32, f32, f32, i32, p, p],
         "cg_mlp_to_bf16": [p, i64, p, p],
         "cg_mlp_decode": [p, i64, i32, p, p, i32, f64, p, i64, p, p, p],
-        "cg_mlp_compact": [p, i64, p, p, p],
+        "cg_mlp_compact": [p, i64, i32, p, p, p, p],
+        "cg_mlp_audit": [p, i64, p, i64, i64, i32, p, f64, f64, p, p, p, p, p],
+        "cg_mlp_fallback_plan": [p, p, i64, f64, i32, p, p, p, p, p],
+        "cg_count_tiles": [p, i64, i32, p, p, p],
         "cg_mlp_gather": [p, p, p, i64, p, p, i64, i32, p, p, p, i64, p],
         "cg_mlp_scatter": [p, i64, p, p, i64, i32, p, i64, p],
         "cg_mlp_pack": [p, i32, i32, i32, i32, p, p],
-        "cg_mlp_infer": [p, p, p, i64, i64, i32, p, p, p, i32, i32, i32, p, p, i32, f64, p, i64, p, p, p],
+        "cg_mlp_infer": [p, p, p, i64, i64, i32, p, p, p, i32, i32, i32, p, p, i32, f64, p, i64, p, p, p, p],
     }
     for name, a in sig.items():
         f = getattr(L, name)
@@ -269,14 +272,16 @@ class Surrogate:
         self._packed = (Wf, bias)
         return self._packed
 
-    def predict_(self, P, T, Y, W, tol: float = 1e-3, stream=None, fused: Optional[bool] = None):
-        """Surrogate wdot into W [Ns, ld] (FP64) plus per-point flags/residuals.  ``fused``
-        (default: whenever the shape allows) runs the one-launch inference kernel; otherwise
-        the layer-by-layer products of the training path are used."""
+    def predict_(self, P, T, Y, W, tol: float = 1e-3, stream=None, fused: Optional[bool] = None,
+                 flag=None, resid=None, skip=None):
+        """Surrogate wdot into W [Ns, ld] (FP64) plus per-point flags/residuals (into ``flag`` /
+        ``resid`` when given).  ``fused`` (default: whenever the shape allows) runs the
+        one-launch inference kernel, which ``skip`` (a device int32, fused only) can cancel on
+        the device; otherwise the layer-by-layer products of the training path are used."""
         import torch
         ng = P.shape[0]
-        flag = torch.empty(ng, dtype=torch.int32, device=self.dev)
-        resid = torch.empty(ng, dtype=torch.float64, device=self.dev)
+        flag = torch.empty(ng, dtype=torch.int32, device=self.dev) if flag is None else flag
+        resid = torch.empty(ng, dtype=torch.float64, device=self.dev) if resid is None else resid
         ld = W.stride(0) if W.shape[0] > 1 else W.shape[1]
         if fused is None:
             fused = self.fused_ok
@@ -288,7 +293,8 @@ class Surrogate:
             check(lib().cg_mlp_infer(P.data_ptr(), T.data_ptr(), Y.data_ptr(), ldy, ng, self.ns,
                                      self.fscale.data_ptr(), Wf.data_ptr(), bias.data_ptr(), self.dims[1], self.kin,
                                      len(self.dims) - 2, self.s.data_ptr(), self.aek.data_ptr(), self.aek.shape[0],
-                                     tol, W.data_ptr(), ld, flag.data_ptr(), resid.data_ptr(), _s(stream)),
+                                     tol, W.data_ptr(), ld, flag.data_ptr(), resid.data_ptr(), _ptr(skip),
+                                     _s(stream)),
                   "cg_mlp_infer")
             return flag, resid
         X = self.features(P, T, Y, stream)
@@ -298,83 +304,154 @@ class Surrogate:
                                   _s(stream)), "cg_mlp_decode")
         return flag, resid
 
-    def _gather(self, P, T, Y, idx, count, n):
-        """States of the points idx[:n] packed into new [n] / [Ns, n] tensors."""
+    def _gather(self, P, T, Y, idx, count, n, out=None, stream=None):
+        """States of the points idx[:*count] (at most n) packed into [n] / [Ns, n] tensors (new,
+        or the three of ``out``)."""
         import torch
-        Pp = torch.empty(n, dtype=torch.float64, device=self.dev)
-        Tp = torch.empty_like(Pp)
-        Yp = torch.empty(self.ns, n, dtype=torch.float64, device=self.dev)
+        if out is None:
+            Pp = torch.empty(n, dtype=torch.float64, device=self.dev)
+            Tp = torch.empty_like(Pp)
+            Yp = torch.empty(self.ns, n, dtype=torch.float64, device=self.dev)
+        else:
+            Pp, Tp, Yp = out
         ldy = Y.stride(0) if Y.shape[0] > 1 else Y.shape[1]
         check(lib().cg_mlp_gather(P.data_ptr(), T.data_ptr(), Y.data_ptr(), ldy, idx.data_ptr(), count.data_ptr(),
-                                  n, self.ns, Pp.data_ptr(), Tp.data_ptr(), Yp.data_ptr(), n, _s(None)),
+                                  n, self.ns, Pp.data_ptr(), Tp.data_ptr(), Yp.data_ptr(), Yp.shape[1], _s(stream)),
               "cg_mlp_gather")
         return Pp, Tp, Yp
 
-    def _scatter(self, Wp, idx, count, n, W):
+    def _scatter(self, Wp, idx, count, n, W, stream=None):
         ldw = W.stride(0) if W.shape[0] > 1 else W.shape[1]
-        check(lib().cg_mlp_scatter(Wp.data_ptr(), n, idx.data_ptr(), count.data_ptr(), n, self.ns,
-                                   W.data_ptr(), ldw, _s(None)), "cg_mlp_scatter")
+        check(lib().cg_mlp_scatter(Wp.data_ptr(), Wp.shape[1], idx.data_ptr(), count.data_ptr(), n, self.ns,
+                                   W.data_ptr(), ldw, _s(stream)), "cg_mlp_scatter")
 
     def error(self, Wsur, Wref):
         """Per-point surrogate error max_k |w_sur - w| / (|w| + s_k) (the accuracy measure)."""
         return ((Wsur - Wref).abs() / (Wref.abs() + self.s[:, None])).max(dim=0).values
 
-    def getrates_(self, P, T, Y, W, fallback: DeviceMechanism, tol: float = 1e-3, audit_every: int = 64,
-                  audit_max_bad: float = 0.05, dense_frac: float = 0.5) -> dict:
-        """wdot into W via surrogate + per-point FP64 fallback; returns what fell back.
+    def _workspace(self, ng: int, audit_every: int, tile: int):
+        """Device buffers of getrates_ for ng points, allocated once per (ng, audit_every, tile)."""
+        import torch
+        key = (ng, audit_every, tile)
+        ws = getattr(self, "_ws", None)
+        if ws is not None and ws["key"] == key:
+            return ws
+        d = self.dev
+        i32 = lambda n: torch.zeros(n, dtype=torch.int32, device=d)
+        nt = (ng + tile - 1) // tile
+        na = (ng + audit_every - 1) // audit_every if audit_every > 0 else 0
+        ws = {"key": key, "na": na, "flag": i32(ng), "resid": torch.empty(ng, dtype=torch.float64, device=d),
+              "reject": i32(1), "n_bad": i32(1), "count": i32(1), "full": i32(1), "pcount": i32(1),
+              "acc": i32(2),      # the audit's ticket counters, left zero by each call
+              "idx": torch.zeros(ng, dtype=torch.int64, device=d),
+              "packed": (torch.empty(ng, dtype=torch.float64, device=d), torch.empty(ng, dtype=torch.float64, device=d),
+                         torch.empty(self.ns, ng, dtype=torch.float64, device=d)),
+              "Wp": torch.empty(self.ns, ng, dtype=torch.float64, device=d),
+              "ts": torch.arange(0, nt * tile, tile, dtype=torch.int64, device=d), "tl_p": i32(nt), "tl_f": i32(nt),
+              "side": torch.cuda.Stream(device=d)}
+        if na:
+            ws["aidx"] = torch.arange(0, ng, audit_every, dtype=torch.int64, device=d)
+            ws["acount"] = torch.full((1,), na, dtype=torch.int32, device=d)
+            ws["apacked"] = (torch.empty(na, dtype=torch.float64, device=d), torch.empty(na, dtype=torch.float64, device=d),
+                             torch.empty(self.ns, na, dtype=torch.float64, device=d))
+            ws["Wa"] = torch.empty(self.ns, na, dtype=torch.float64, device=d)
+            ws["Wsa"] = torch.empty(self.ns, na, dtype=torch.float64, device=d)
+            ws["aflag"] = i32(na)
+            ws["aresid"] = torch.empty(na, dtype=torch.float64, device=d)
+        self._ws = ws
+        return ws
 
-        1. Audit first: every ``audit_every``-th point is evaluated by both the surrogate and
-           the FP64 kernel.  If more than ``audit_max_bad`` of them are off by more than 1 %,
-           the whole batch goes to the FP64 kernel and the network is not run on the rest.
-        2. Otherwise the surrogate fills W; the audited points get their FP64 values; every
-           other point whose element-conservation residual exceeds ``tol`` (or is not finite)
-           is recomputed in FP64 -- packed into a dense sub-batch, or, when more than
-           ``dense_frac`` of the points need it, by one FP64 launch over the whole batch.
-        Runs on the current stream (the FP64 launches too) and synchronises with the host
-        twice (audit verdict, fallback count)."""
+    def getrates_(self, P, T, Y, W, fallback: DeviceMechanism, tol: float = 1e-3, audit_every: Optional[int] = None,
+                  audit_max_bad: float = 0.05, dense_frac: float = 0.5, stream=None) -> dict:
+        """wdot into W via surrogate + per-point FP64 fallback, with no host synchronisation.
+
+        1. Audit: every ``audit_every``-th point (default max(16, ng // 4096), about 4096
+           points) is evaluated by the network and, on a side stream at the same time, by the
+           FP64 kernel; the device counts the audited points off by more than 1 % and sets
+           ``reject`` if they are more than ``audit_max_bad`` of them (cg_mlp_audit).
+        2. The network runs over the batch unless ``reject`` (cg_mlp_infer reads the flag and
+           returns at once); the audited points get their FP64 values.
+        3. The points whose element-conservation residual exceeds ``tol`` (or is not finite) are
+           compacted on the device; if the audit rejected the network or more than
+           ``dense_frac`` of the points are flagged, the whole batch goes to one FP64 launch,
+           else the flagged points are recomputed packed (cg_mlp_compact,
+           cg_mlp_fallback_plan).  Both FP64 launches run over tile lists whose lengths follow
+           those device counts
+           (cg_mlp_fallback_plan, cg_getrates_tiles), so the launch sequence is fixed and the
+           whole path can be captured into a HIP graph (after one eager call with the same ng,
+           which allocates the workspace).
+        Only the fused inference kernel can be cancelled on the device: networks it does not
+        cover run layer by layer even when rejected (the FP64 launch then overwrites them).
+        Runs on ``stream`` (default: the current stream).  Returns the device counters;
+        ``fallback_info`` turns them into numbers (that synchronises)."""
+        import contextlib
         import torch
         ng = P.shape[0]
-        info = {"fallback_points": 0, "audit_points": 0, "audit_bad_frac": 0.0, "audit_err_median": None,
-                "audit_err_max": None, "global_fallback": False}
         if ng == 0:
-            return info
-        aidx = acount = Wa = None
-        if audit_every > 0:
-            aidx = torch.arange(0, ng, audit_every, dtype=torch.int64, device=self.dev)
-            na = int(aidx.numel())
-            acount = torch.full((1,), na, dtype=torch.int32, device=self.dev)
-            Pa, Ta, Ya = self._gather(P, T, Y, aidx, acount, na)
-            Wa = torch.empty_like(Ya)
-            fallback.getrates_(Pa, Ta, Ya, Wa)
-            Wsa = torch.empty_like(Ya)
-            self.predict_(Pa, Ta, Ya, Wsa, tol)
-            err = self.error(Wsa, Wa)
-            bad = float((~(err <= 1e-2)).double().mean().item())
-            info.update(audit_points=na, audit_bad_frac=bad, audit_err_median=float(err.median().item()),
-                        audit_err_max=float(err.max().item()))
-            if bad > audit_max_bad:
-                fallback.getrates_(P, T, Y, W)
-                info.update(fallback_points=ng, global_fallback=True)
-                return info
-        flag, _ = self.predict_(P, T, Y, W, tol)
-        if aidx is not None:
-            self._scatter(Wa, aidx, acount, int(aidx.numel()), W)
-            flag[::audit_every] = 0
-        count = torch.zeros(1, dtype=torch.int32, device=self.dev)
-        idx = torch.empty(ng, dtype=torch.int64, device=self.dev)
-        check(lib().cg_mlp_compact(flag.data_ptr(), ng, count.data_ptr(), idx.data_ptr(), _s(None)),
-              "cg_mlp_compact")
-        nfb = int(count.item())
-        if nfb > dense_frac * ng:
-            fallback.getrates_(P, T, Y, W)
-            info["fallback_points"] = ng
-        elif nfb:
-            Pp, Tp, Yp = self._gather(P, T, Y, idx, count, nfb)
-            Wp = torch.empty_like(Yp)
-            fallback.getrates_(Pp, Tp, Yp, Wp)
-            self._scatter(Wp, idx, count, nfb, W)
-            info["fallback_points"] = nfb
-        return info
+            return {"ng": 0}
+        if ng > 0x7fffffff:
+            raise ChemgenError("Surrogate.getrates_: at most 2^31 - 1 points per call")
+        if audit_every is None:
+            audit_every = max(16, ng // 4096)
+        # the torch work below runs on the current stream: make it ``stream``
+        with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            return self._getrates_dev(P, T, Y, W, fallback, tol, audit_every, audit_max_bad, dense_frac)
+
+    def _getrates_dev(self, P, T, Y, W, fallback, tol, audit_every, audit_max_bad, dense_frac):
+        import torch
+        ng = P.shape[0]
+        # only the fused kernel can be cancelled on the device; the layer-by-layer path (networks
+        # it does not cover) runs even when the audit rejects, and the whole-batch FP64 launch then
+        # overwrites its output
+        fused = self.fused_ok
+        tile = fallback.block
+        ws = self._workspace(ng, audit_every, tile)
+        cur = torch.cuda.current_stream()
+        st = int(cur.cuda_stream)
+        na = ws["na"]
+        if na:
+            Pa, Ta, Ya = self._gather(P, T, Y, ws["aidx"], ws["acount"], na, out=ws["apacked"], stream=cur)
+            side = ws["side"]
+            side.wait_stream(cur)
+            fallback.getrates_(Pa, Ta, Ya, ws["Wa"], stream=side)
+            self.predict_(Pa, Ta, Ya, ws["Wsa"], tol, stream=cur, fused=fused, flag=ws["aflag"], resid=ws["aresid"])
+            cur.wait_stream(side)
+            check(lib().cg_mlp_audit(ws["Wsa"].data_ptr(), na, ws["Wa"].data_ptr(), na, na, self.ns,
+                                     self.s.data_ptr(), 1e-2, audit_max_bad, ws["reject"].data_ptr(),
+                                     ws["n_bad"].data_ptr(), ws["acc"].data_ptr(), ws["count"].data_ptr(), st),
+                  "cg_mlp_audit")
+        else:
+            ws["reject"].zero_()
+            ws["count"].zero_()
+        self.predict_(P, T, Y, W, tol, stream=cur, fused=fused, flag=ws["flag"], resid=ws["resid"],
+                      skip=ws["reject"] if fused else None)
+        if na:
+            self._scatter(ws["Wa"], ws["aidx"], ws["acount"], na, W, stream=cur)
+        check(lib().cg_mlp_compact(ws["flag"].data_ptr(), ng, audit_every, ws["count"].data_ptr(),
+                                   ws["idx"].data_ptr(), ws["reject"].data_ptr(), st), "cg_mlp_compact")
+        check(lib().cg_mlp_fallback_plan(ws["count"].data_ptr(), ws["reject"].data_ptr(), ng, dense_frac, tile,
+                                         ws["full"].data_ptr(), ws["pcount"].data_ptr(), ws["tl_p"].data_ptr(),
+                                         ws["tl_f"].data_ptr(), st), "cg_mlp_fallback_plan")
+        # packed recompute of the flagged points, on a bounded grid (the tile kernel strides over
+        # the list, so an empty list costs one short launch)
+        Pp, Tp, Yp = self._gather(P, T, Y, ws["idx"], ws["pcount"], ng, out=ws["packed"], stream=cur)
+        fallback.getrates_tiles_(Pp, Tp, Yp, ws["Wp"], ws["ts"], ws["tl_p"], ng, grid=1024, stream=cur)
+        self._scatter(ws["Wp"], ws["idx"], ws["pcount"], ng, W, stream=cur)
+        # the whole batch in FP64 (zero-length tiles unless the plan says so)
+        fallback.getrates_tiles_(P, T, Y, W, ws["ts"], ws["tl_f"], ng, grid=-1, stream=cur)
+        return {"ng": ng, "na": na, "reject": ws["reject"], "n_bad": ws["n_bad"], "count": ws["pcount"],
+                "full": ws["full"]}
+
+    @staticmethod
+    def fallback_info(d: dict) -> dict:
+        """Host numbers of a getrates_ result (synchronises with the device)."""
+        if d.get("ng", 0) == 0:
+            return {"fallback_points": 0, "audit_points": 0, "audit_bad_frac": 0.0, "global_fallback": False}
+        full = int(d["full"].item())
+        na = d["na"]
+        return {"fallback_points": full if full else int(d["count"].item()),
+                "audit_points": na, "audit_bad_frac": int(d["n_bad"].item()) / max(1, na),
+                "global_fallback": bool(full), "audit_rejected": bool(int(d["reject"].item()))}
 
     def state_dict(self) -> dict:
         return {"dims": self.dims, "fscale": self.fscale.cpu().numpy(), "s": self.s.cpu().numpy(),

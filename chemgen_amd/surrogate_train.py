@@ -2,6 +2,8 @@
 
     python -m chemgen_amd.surrogate_train --mech synth_gri53 --n-train 8388608 --epochs 12 \
         --out chemgen_amd/data/surrogate_synth_gri53.npz --report report.json
+    python -m chemgen_amd.surrogate_train --mech synth_gri53 --sweep 256x2,256x4,512x4 \
+        --n-train 8388608 --epochs 8 --report sweep.json
 
 Training states: T U(800, 2800) K, P 1-40 atm log-uniform, Y Dirichlet(0.5) (seeded, drawn on
 the GPU), labelled by the FP64 getrates kernel.  Evaluation: 2^20 held-out states (another
@@ -55,7 +57,7 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
     flag, resid = sur.predict_(P, T, Y, W, tol)
     err_raw = sur.error(W, ref)
     floor = bf16_floor(sur, P, T, Y, W)
-    info = sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every)
+    info = sur.fallback_info(sur.getrates_(P, T, Y, W, dm, tol=tol, audit_every=audit_every))
     err = sur.error(W, ref)
     torch.cuda.synchronize()
 
@@ -101,6 +103,82 @@ def evaluate(sur, dm, n_eval: int, seed: int, tol: float = 1e-3, audit_every: in
     }
 
 
+def sweep(mech_name: str, shapes, n_train: int, epochs: int, n_eval: int = 1 << 20, batch: int = 8192,
+          lr: float = 2e-3, log=print) -> list:
+    """Accuracy against speed over network shapes ((hidden, layers) pairs), each trained with the
+    same data and budget: per shape the fraction of held-out points within 1% and the median
+    error of the fused BF16 network, the same for its FP32 master weights evaluated in FP64 (what
+    any wider-precision evaluation of this network could reach at most), and the points/s of the
+    fused inference kernel against the FP64 kernel on the same points."""
+    import numpy as np
+    import torch
+    from .autotune import best_variant
+    from .kernels import load_bundled
+    from .runtime import DeviceMechanism
+    from .surrogate import Surrogate, random_states
+    mech = load_bundled(mech_name)
+    dm = DeviceMechanism(mech, best_variant(mech), build=False)
+    P, T, Y = random_states(mech.n_species, n_train, seed=100)
+    W = torch.empty_like(Y)
+    dm.getrates_(P, T, Y, W)
+    Pe, Te, Ye = random_states(mech.n_species, n_eval, seed=7)
+    ref = torch.empty_like(Ye)
+    dm.getrates_(Pe, Te, Ye, ref)
+    torch.cuda.synchronize()
+    out = []
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t64 = timed(lambda: dm.getrates_(Pe, Te, Ye, ref))
+    for hidden, layers in shapes:
+        sur = Surrogate(mech, hidden=hidden, n_hidden=layers, seed=0)
+        sur.fit_scales(W)
+        t0 = time.perf_counter()
+        sur.train(P, T, Y, W, epochs=epochs, batch=batch, lr=lr,
+                  log=lambda m: log(f"[sweep {hidden}x{layers}] {m}"))
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        Ws = torch.empty_like(Ye)
+        sur.predict_(Pe, Te, Ye, Ws)
+        err = sur.error(Ws, ref)
+        # the same weights in FP64 on a sample: features -> FP32 masters in FP64 -> decode
+        n = 8192
+        X = sur.features(Pe[:n], Te[:n], Ye[:, :n].contiguous()).double().cpu().numpy()
+        h = X
+        for l, L in enumerate(sur.layers):
+            h = h @ L.w.double().cpu().numpy().T + L.b.double().cpu().numpy()
+            if l < len(sur.layers) - 1:
+                h = h / (1.0 + np.exp(-h))
+        s = sur.s.cpu().numpy()
+        w64 = (s[None, :] * np.sinh(h)).T
+        r = ref[:, :n].cpu().numpy()
+        e64 = np.max(np.abs(w64 - r) / (np.abs(r) + s[:, None]), axis=0)
+        t_inf = timed(lambda: sur.predict_(Pe, Te, Ye, Ws))
+        flops = 2.0 * sum(sur.dims[i] * sur.dims[i + 1] for i in range(len(sur.dims) - 1))
+        row = {"hidden": hidden, "layers": layers, "params": int(sum(L.w.numel() + L.b.numel() for L in sur.layers)),
+               "flops_per_point": flops, "train_s": t_train, "epochs": epochs, "n_train": n_train,
+               "bf16": {"frac_le_1pct": float((err <= 1e-2).double().mean()), "median_err": float(err.median()),
+                        "frac_le_10pct": float((err <= 1e-1).double().mean())},
+               "fp64_eval": {"points": n, "frac_le_1pct": float(np.mean(e64 <= 1e-2)),
+                             "median_err": float(np.median(e64)), "frac_le_10pct": float(np.mean(e64 <= 1e-1))},
+               "inference_points_per_s": n_eval / t_inf, "fp64_points_per_s": n_eval / t64,
+               "inference_vs_fp64": t64 / t_inf, "inference_tflops": flops * n_eval / t_inf / 1e12}
+        log("[sweep] " + json.dumps(row))
+        out.append(row)
+        del sur, Ws
+        torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None):
     import numpy as np
     import torch
@@ -120,7 +198,17 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="weights (.npz)")
     ap.add_argument("--report", default=None, help="JSON report")
     ap.add_argument("--load", default=None, help="skip training, load weights")
+    ap.add_argument("--sweep", default=None,
+                    help="accuracy against speed over network shapes, e.g. 256x2,256x4,512x4 (same data and epochs)")
     a = ap.parse_args(argv)
+    if a.sweep:
+        shapes = [tuple(int(v) for v in t.split("x")) for t in a.sweep.split(",")]
+        rows = sweep(a.mech, shapes, a.n_train, a.epochs, a.n_eval, a.batch, a.lr,
+                     log=lambda m: print(m, flush=True))
+        if a.report:
+            with open(a.report, "w") as fh:
+                json.dump({"mechanism": a.mech, "sweep": rows}, fh, indent=1)
+        return
     mech = load_bundled(a.mech)
     dm = DeviceMechanism(mech, best_variant(mech), build=False)
     sur = Surrogate(mech, hidden=a.hidden, n_hidden=a.layers, seed=0)

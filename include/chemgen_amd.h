@@ -115,6 +115,15 @@ int cg_getrates_batched_prepared(cg_mech_t h, const double* P, const double* T,
                                  const double* Y, double* wdot, int64_t ld, int32_t grid,
                                  void* stream);
 
+/* One launch of the mechanism's tile kernel over a DEVICE tile list the caller owns:
+ * tile t covers points [tile_start[t], tile_start[t] + tile_len[t]) (length at most the
+ * kernel's points per workgroup, inside [0, n_points) <= ld); tiles of length 0 are skipped, so
+ * a list whose lengths are set on the device (cg_count_tiles) makes a launch whose point count
+ * is a device value.  No counterpart in the reference. */
+int cg_getrates_tiles(cg_mech_t h, const double* P, const double* T, const double* Y, double* wdot,
+                      int64_t ld, const int64_t* tile_start, const int32_t* tile_len, int64_t n_tiles,
+                      int64_t n_points, int32_t grid, void* stream);
+
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
  * cg_getrates_batched without any allocation.  The tile list is built on the device into a
@@ -163,8 +172,31 @@ int cg_mlp_to_bf16(const float* x, int64_t n, void* y, void* stream);
 int cg_mlp_decode(const float* Zh, int64_t ng, int32_t ns, const double* s, const float* aek,
                   int32_t ne, double tol, double* W, int64_t ld, int32_t* flag, double* resid,
                   void* stream);
-/* idx[(*count)++] = m for every flagged m (order unspecified) */
-int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t* count, int64_t* idx, void* stream);
+/* idx[(*count)++] = m for every flagged m that is not audited (m % audit_every != 0; every
+ * flagged m when audit_every <= 0), in unspecified order; nothing while *skip != 0 (skip may be
+ * NULL).  One atomic add on *count per 4096 points. */
+int cg_mlp_compact(const int32_t* flag, int64_t ng, int32_t audit_every, int32_t* count, int64_t* idx,
+                   const int32_t* skip, void* stream);
+/* The surrogate audit's verdict on the device: *n_bad = number of the n audited points with
+ * max_k |Ws - Wr| / (|Wr| + s_k) > err_tol or not finite (an integer count, so deterministic),
+ * *reject = (*n_bad > max_bad * n), and *count_clear = 0 when count_clear is not NULL (the
+ * compaction's counter).  acc: two uint32 of device workspace, zero before the first call; each
+ * call leaves them zero. */
+int cg_mlp_audit(const double* Ws, int64_t lds, const double* Wr, int64_t ldr, int64_t n, int32_t ns,
+                 const double* s, double err_tol, double max_bad, int32_t* reject, int32_t* n_bad,
+                 uint32_t* acc, int32_t* count_clear, void* stream);
+/* The fallback plan after the compaction, and the tile lengths of its two FP64 launches over
+ * [0, ng) in tiles of `tile` points (tile t starts at t * tile): if *reject or
+ * *count > dense_frac * ng, *full = ng and *pcount = 0 (one launch over the whole batch), else
+ * *full = 0 and *pcount = *count (the flagged points recomputed packed);
+ * len_packed[t] = clamp(*pcount - t tile, 0, tile), len_full[t] = clamp(*full - t tile, 0, tile). */
+int cg_mlp_fallback_plan(const int32_t* count, const int32_t* reject, int64_t ng, double dense_frac,
+                         int32_t tile, int32_t* full, int32_t* pcount, int32_t* len_packed,
+                         int32_t* len_full, void* stream);
+/* Device tile list for the first *count points of [0, cap): tile_start[t] = t * tile,
+ * tile_len[t] = clamp(*count - t * tile, 0, tile), t < ceil(cap / tile). */
+int cg_count_tiles(const int32_t* count, int64_t cap, int32_t tile, int64_t* tile_start,
+                   int32_t* tile_len, void* stream);
 /* pack / unpack the listed points (first *count of idx; cap = launch bound) */
 int cg_mlp_gather(const double* P, const double* T, const double* Y, int64_t ld, const int64_t* idx,
                   const int32_t* count, int64_t cap, int32_t ns, double* Po, double* To, double* Yo,
@@ -179,11 +211,12 @@ int cg_mlp_pack(const void* W, int32_t N, int32_t K, int32_t NT, int32_t KC, voi
  * residual/flag as cg_mlp_decode.  Wf: the layers packed by cg_mlp_pack back to back
  * (NT = hidden/16 for hidden layers, ceil(ns/16) for the output; KC = ceil(kin/32) for the
  * first layer, hidden/32 after); bias: FP32, hidden values per hidden layer then ns.
- * Requires ns + 2 <= kin <= hidden, ns <= hidden, ne <= 8. */
+ * Requires ns + 2 <= kin <= hidden, ns <= hidden, ne <= 8.  skip (device, may be NULL): when *skip != 0
+ * at launch, every workgroup returns at once (a device-side verdict cancels the launch). */
 int cg_mlp_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng, int32_t ns,
                  const float* fscale, const void* Wf, const float* bias, int32_t hidden, int32_t kin,
                  int32_t n_hidden, const double* s, const float* aek, int32_t ne, double tol, double* W,
-                 int64_t ldw, int32_t* flag, double* resid, void* stream);
+                 int64_t ldw, int32_t* flag, double* resid, const int32_t* skip, void* stream);
 
 /* ---- measurement (no counterpart in the reference) ---- */
 

@@ -75,7 +75,16 @@ def test_surrogate_entry_points_reject_bad_arguments_without_gpu():
         ("cg_mlp_to_bf16", (x, -1, x, None)),
         ("cg_mlp_decode", (x, -1, 4, x, x, 2, 1e-2, x, 8, x, x, None)),
         ("cg_mlp_decode", (x, 8, 4, x, x, 2, 1e-2, x, 8, None, x, None)),
-        ("cg_mlp_compact", (x, -1, x, x, None)),
+        ("cg_mlp_compact", (x, -1, 64, x, x, None, None)),
+        ("cg_mlp_compact", (None, 8, 64, x, x, None, None)),
+        ("cg_mlp_audit", (x, 4, x, 4, 8, 4, x, 1e-2, 0.05, x, x, x, None, None)),   # leading dimension below n
+        ("cg_mlp_audit", (x, 8, x, 8, 8, 4, x, 1e-2, 0.05, None, x, x, None, None)),
+        ("cg_mlp_audit", (x, 8, x, 8, 8, 4, x, 1e-2, 0.05, x, x, None, None, None)),  # no workspace
+        ("cg_mlp_fallback_plan", (x, x, 0, 0.5, 64, x, x, x, x, None)),
+        ("cg_mlp_fallback_plan", (x, x, 8, 0.5, 0, x, x, x, x, None)),
+        ("cg_mlp_fallback_plan", (x, None, 8, 0.5, 64, x, x, x, x, None)),
+        ("cg_count_tiles", (x, 8, 0, x, x, None)),
+        ("cg_count_tiles", (None, 8, 64, x, x, None)),
         ("cg_mlp_gather", (x, x, x, 8, x, x, -1, 4, x, x, x, 8, None)),
         ("cg_mlp_gather", (x, x, x, 8, x, x, 8, 4, x, x, x, 4, None)),
         ("cg_mlp_scatter", (x, 8, x, x, -1, 4, x, 8, None)),
