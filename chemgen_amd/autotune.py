@@ -27,9 +27,24 @@ STRAIGHT_LINE_MAX_SPECIES = 150  # above this the lane-per-point and split kerne
 
 
 def table_lds_bytes(ns: int, npt: int, waves: int) -> int:
-    """LDS of one table-kernel tile: EG, h, wdot [ns + 1][npt] (one row of ones for absent
-    species slots) and the per-wavefront partial sums."""
-    return 8 * (3 * (ns + 1) * npt + waves * npt)
+    """Least LDS of one table-kernel tile: EG, h [ns + 1][npt] (one row of ones for absent
+    species slots), the per-wavefront partial sums, and the rates of progress of one round of
+    reaction slots (waves x 64 lanes); the kernel gives its chunk buffer the rest of the LDS."""
+    return 8 * (2 * (ns + 1) * npt + waves * npt + waves * 64)
+
+
+def table_chunk_reactions(ns: int, nr: int, npt: int, waves: int) -> int:
+    """Reactions whose rates of progress the table kernel holds at once (its chunk): the LDS
+    left after EG, h, the partial sums and the zero row, in rows of npt doubles."""
+    return min(nr, (LDS_BYTES // 8 - (2 * (ns + 1) + waves + 1) * npt) // npt)
+
+
+def table_fits(ns: int, nr: int, npt: int, waves: int, min_chunk: int = 0) -> bool:
+    """Whether the table kernel of npt-point tiles and ``waves`` wavefronts holds the mechanism:
+    its LDS, the 16-bit LDS row offsets of the reaction table (ns npt 8 < 2^16), and a chunk
+    of at least min(nr, min_chunk) reactions."""
+    return (table_lds_bytes(ns, npt, waves) <= LDS_BYTES and ns * npt * 8 < 1 << 16
+            and table_chunk_reactions(ns, nr, npt, waves) >= min(nr, min_chunk))
 
 
 def candidate_variants(mech: Mechanism) -> List[Variant]:
@@ -61,10 +76,11 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
                         Variant(64, wpe, "lean", "gh", "horner", False, 4, 1, 0, True, True)]
     if table_supported(mech):
         # tiles shorter than 16 points only where a 16-point tile does not fit
-        short = (1, 2, 4, 8) if table_lds_bytes(ns, 16, 4) > LDS_BYTES else ()
+        nr = mech.n_reactions
+        short = (1, 2, 4, 8) if not table_fits(ns, nr, 16, 4, 512) else ()
         for npt in short + (16, 32, 64):
             for waves, wpe in ((4, 1), (8, 2), (16, 4)):
-                if table_lds_bytes(ns, npt, waves) <= LDS_BYTES:
+                if table_fits(ns, nr, npt, waves):
                     out.append(Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt))
     return out
 
@@ -94,8 +110,9 @@ def default_variant(mech: Mechanism) -> Variant:
     species (two tiles per CU), 8 above; otherwise the table kernel (a mechanism such as Hashemi
     2016, 68 species with 114 PLOG reactions of 908 terms, spills 2-9 KB per lane in the
     straight-line kernels and runs 1.4-3.6x faster in the table kernel), with the longest tile of
-    at most 16 points whose species state fits the LDS and the most wavefronts that fit with it
-    (16, 4 per SIMD, won at every tile length on the JetSurf-sized mechanism and Fuller 2021)."""
+    at most 16 points whose species state fits the LDS with a chunk of at least 512 reactions
+    (or all of them), and the most wavefronts that fit with it (16, 4 per SIMD, won at every
+    tile length on the JetSurf-sized mechanism and Fuller 2021)."""
     ns = mech.n_species
     if ns <= 24 or getattr(mech, "qss", ()):
         return Variant()
@@ -105,7 +122,11 @@ def default_variant(mech: Mechanism) -> Variant:
         return Variant(64, 2, "lean", "gh", "horner", False, 8, 8)
     for npt in (16, 8, 4, 2, 1):
         for waves, wpe in ((16, 4), (8, 2), (4, 1)):
-            if table_lds_bytes(ns, npt, waves) <= LDS_BYTES:
+            if table_fits(ns, mech.n_reactions, npt, waves, 512):
+                return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt)
+    for npt in (16, 8, 4, 2, 1):
+        for waves, wpe in ((16, 4), (8, 2), (4, 1)):
+            if table_fits(ns, mech.n_reactions, npt, waves):
                 return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt)
     raise ValueError(f"{ns} species: the table kernel's one-point tile does not fit the LDS")
 

@@ -1085,6 +1085,269 @@ def table_layout(mech: Mechanism, npt: int = 16, min_class: int = 0) -> TableLay
     return TableLayout(rows, effsp, order, segs, gen_start, plog, xsl, xnu)
 
 
+def table_chunks(spans: List[Tuple[int, int]], rc: int, slots: int) -> List[Tuple[int, int, List[Tuple[int, int, int]]]]:
+    """Split the table's reaction ranges (``spans``: the compiled classes then the generic
+    range, contiguous in table order) into chunks of at most ``rc`` reactions, whose rates of
+    progress fit the table kernel's LDS buffer at once.  Returns [(c0, c1, pieces)], pieces =
+    [(a, b, span index)] covering [c0, c1).  Chunks are filled to rc (ceil(total / rc) of
+    them): a range cut at a chunk end leaves a part-empty round there, which costs less than
+    the chunk more that cuts after whole rounds would need (a gather and two barriers)."""
+    total = spans[-1][1] if spans else 0
+    if rc < min(slots, total):
+        raise ValueError(f"table kernel: {rc} reactions per chunk is less than one round of {slots} slots")
+    chunks: List[Tuple[int, int, List[Tuple[int, int, int]]]] = []
+    c0, pieces = 0, []
+    for si, (a, b) in enumerate(spans):
+        while a < b:
+            room = c0 + rc - a
+            if room == 0:
+                chunks.append((c0, a, pieces))
+                c0, pieces, room = a, [], rc
+            take = min(b - a, room)
+            pieces.append((a, a + take, si))
+            a += take
+    if pieces:
+        chunks.append((c0, spans[-1][1] if spans else 0, pieces))
+    return chunks
+
+
+def table_incidence(mech: Mechanism, order: List[int], chunks, npt: int):
+    """Per chunk and species, the reactions of the chunk that change the species: CSR offsets
+    goff[c (ns + 1) + k], and per entry the row of the rate of progress in the chunk's LDS
+    buffer (in doubles: (j - c0) npt) and the net coefficient nu'' - nu' (entries in table
+    order; species on both sides with equal coefficients get none)."""
+    ns = mech.n_species
+    idx = {s.name: i for i, s in enumerate(mech.species)}
+    goff: List[int] = []
+    grow: List[int] = []
+    gnu: List[float] = []
+    for (c0, c1, _) in chunks:
+        lists: List[List[Tuple[int, float]]] = [[] for _ in range(ns)]
+        for j in range(c0, c1):
+            r = mech.reactions[order[j]]
+            net: Dict[int, float] = {}
+            for sp, nu in r.reactants.items():
+                net[idx[sp]] = net.get(idx[sp], 0.0) - float(nu)
+            for sp, nu in r.products.items():
+                net[idx[sp]] = net.get(idx[sp], 0.0) + float(nu)
+            for k, nu in net.items():
+                if nu != 0.0:
+                    lists[k].append(((j - c0) * npt, nu))
+        for k in range(ns):
+            goff.append(len(grow))
+            for row, nu in lists[k]:
+                grow.append(row)
+                gnu.append(nu)
+        goff.append(len(grow))
+    return goff, grow, gnu
+
+
+GATHER_FIRST = 1 << 30     # writer flag: the species' first chunk stores wdot, later chunks add
+
+
+def _gather_order(lens: List[int]) -> List[Tuple[int, int]]:
+    """Order of a lane group's stream words in a chunk: slot by slot (each slot's sum is
+    stored as soon as it is complete, which frees its registers)."""
+    return [(s_, u) for s_ in range(len(lens)) for u in range(lens[s_])]
+
+
+def _gather_coef_word(nu: float):
+    """The bits 18-31 of a gather entry for coefficient ``nu``: the high word of the double,
+    when its low word and the high word's bits 0-17 are zero (the double is then exactly
+    {0, word}); None otherwise."""
+    import struct
+    lo, hi = struct.unpack("<II", struct.pack("<d", float(nu)))
+    return hi if lo == 0 and hi & 0x3ffff == 0 else None
+
+
+def _gather_rows(lens_entries: Dict[int, list], W: int, G: int, thr: float):
+    """Rows of one chunk's gather: a species with more than ``thr`` entries (and more than G)
+    is split over the G lane groups of one wavefront (a combined row); the others are sorted
+    by entry count and taken G at a time.  Rows sorted longest first; slot s of the chunk is
+    rows [W s, W s + W), padded to the length of its first (longest) row."""
+    rows, single = [], []
+    for k, v in lens_entries.items():
+        if len(v) > thr and len(v) > G:
+            part = -(-len(v) // G)
+            rows.append((part, True, [(k if g == 0 else -1, v[g * part:(g + 1) * part]) for g in range(G)]))
+        else:
+            single.append((k, v))
+    single.sort(key=lambda kv: (-len(kv[1]), kv[0]))
+    for i in range(0, len(single), G):
+        grp = single[i:i + G] + [(-1, [])] * (G - len(single[i:i + G]))
+        rows.append((max(len(v) for _, v in grp), False, grp))
+    rows.sort(key=lambda r: -r[0])
+    lens = [rows[i][0] for i in range(0, len(rows), W)]
+    return rows, lens
+
+
+def table_gather_streams(mech: Mechanism, chunks, goff: List[int], grow: List[int], gnu: List[float],
+                         npt: int, W: int, rc: int) -> Dict:
+    """The table kernel's gather, scheduled on the host.  A wavefront's 64 lanes form G = 64 /
+    npt lane groups (one lane per point of the tile).  Chunk c has slots s = 0 .. S_c - 1 of
+    fixed lengths L_s (steps), the same in every wavefront, so the kernel's code for a chunk is
+    straight-line with static register indices.  In slot s of wavefront w each lane group sums
+    the entries nu q of one species (the W x G species of a slot have similar entry counts), or
+    all G groups of w sum parts of one long species and combine them with a butterfly over the
+    groups (bit s of the wavefront's combine mask).  Shorter rows are padded with zero-row
+    entries.  Every species is written by one group of the first chunk with entries for it
+    (untouched species: a one-entry zero term in chunk 0), which stores its wdot; later chunks
+    add to it.  Of the split thresholds 1 and 1/2 times the mean entries per group, the one
+    with fewer steps is used.
+
+    Entry (one 32-bit word): bits 0-17 the LDS byte offset of a rate-of-progress row of the
+    chunk ((j - c0) npt 8; the zero row rc npt 8 pads), bits 18-31 the high word of the net
+    coefficient nu'' - nu' as a double, whose other bits are zero for every integer up to 8
+    and the halves (``_gather_coef_word``); a mechanism with another coefficient writes 0
+    there and the parallel double array ``fnu`` holds every entry's coefficient.
+
+    Returns per chunk c (``chunks``): lens (L_s), steps (sum, padded to a multiple of 4),
+    str_base, spk_base, comb_base; lane group (w, g) reads words
+    stream[str_base + (w G + g) steps + t], writers spk[spk_base + (w G + g) S_c + s]
+    (species | GATHER_FIRST, or -1) and the wavefront's combine mask comb[comb_base + w]."""
+    ns = mech.n_species
+    G = 64 // npt
+    frac = not all(_gather_coef_word(v) is not None for v in gnu)
+    zero_row = rc * npt * 8
+    if zero_row >= 1 << 18:
+        raise ValueError("table kernel: rate-of-progress rows beyond the gather's 18-bit offsets")
+    nch = len(chunks)
+    touched = [any(goff[c * (ns + 1) + k + 1] > goff[c * (ns + 1) + k] for c in range(nch)) for k in range(ns)]
+    first = {}
+    for c in range(nch):
+        for k in range(ns):
+            if goff[c * (ns + 1) + k + 1] > goff[c * (ns + 1) + k] and k not in first:
+                first[k] = c
+    for k in range(ns):
+        first.setdefault(k, 0)
+    out, stream, spk, comb, fnu = [], [], [], [], []
+    for c in range(nch):
+        ents = {}
+        for k in range(ns):
+            a, b = goff[c * (ns + 1) + k], goff[c * (ns + 1) + k + 1]
+            if b > a:
+                ents[k] = [(grow[e] * 8, gnu[e]) for e in range(a, b)]
+            elif c == 0 and not touched[k]:
+                ents[k] = [(zero_row, 0.0)]
+        mean = sum(len(v) for v in ents.values()) / (W * G)
+        best = None
+        for f in (1.0, 0.5):
+            rows, lens = _gather_rows(ents, W, G, max(1.0, mean * f))
+            if best is None or sum(lens) < sum(best[1]):
+                best = (rows, lens)
+        rows, lens = best
+        nsl = len(lens)
+        steps = -(-sum(lens) // 4) * 4
+        rec = dict(lens=lens, steps=steps, str_base=len(stream), spk_base=len(spk), comb_base=len(comb),
+                   adds=[any(r[2][g][0] >= 0 and first[r[2][g][0]] != c for r in rows[W * s_:W * s_ + W]
+                             for g in range(G)) for s_ in range(nsl)])
+        for w in range(W):
+            mask = 0
+            lane_rows = [rows[W * s_ + w] if W * s_ + w < len(rows) else (0, False, [(-1, [])] * G)
+                         for s_ in range(nsl)]
+            for s_, r in enumerate(lane_rows):
+                if r[1]:
+                    mask |= 1 << s_
+            comb.append(mask)
+            for g in range(G):
+                words, fv = [], []
+                for (s_, t) in _gather_order(lens):
+                    v = lane_rows[s_][2][g][1]
+                    off, nu = v[t] if t < len(v) else (zero_row, 0.0)
+                    cw = _gather_coef_word(nu)
+                    words.append(off | (cw if cw is not None else 0))
+                    fv.append(float(nu))
+                for s_, r in enumerate(lane_rows):
+                    k = r[2][g][0]
+                    spk.append(-1 if k < 0 else (k | (GATHER_FIRST if first[k] == c else 0)))
+                words += [zero_row] * (steps - len(words))
+                fv += [0.0] * (steps - len(fv))
+                stream.extend(words)
+                fnu.extend(fv)
+        out.append(rec)
+    if nch and any(m >= 1 << 31 for m in comb):
+        raise ValueError("table kernel: more than 31 gather slots in a chunk")
+    return dict(chunks=out, stream=stream, spk=spk, comb=comb, fnu=fnu if frac else None, G=G)
+
+
+def _gather_code(ci: int, rec: Dict, frac: bool) -> str:
+    """Straight-line gather of chunk ``ci`` for the table kernel (see ``table_gather_streams``):
+    writer words, the first block of the lane group's entry stream, the writers' old wdot
+    (addresses clamped into the arrays, so every load is unconditional), then the slot sums
+    with static register indices, each slot's store right after its last step."""
+    lens, steps = rec["lens"], rec["steps"]
+    nsl = len(lens)
+    L = [f"  {{  // gather of chunk {ci}: {nsl} slot(s) of {', '.join(map(str, lens))} steps",
+         f"    const int lg_ = cg_gw * CG_GG + cg_gg;",
+         f"    const int* sk = cg_gspk + {rec['spk_base']} + lg_ * {nsl};",
+         f"    const int cm = cg_gcomb[{rec['comb_base']} + cg_gw];"]
+    L += [f"    const int k{s_} = sk[{s_}];" for s_ in range(nsl)]
+    L.append(f"    const uint4* st = (const uint4*)(cg_gstr + {rec['str_base']} + lg_ * {steps});")
+    if frac:
+        L.append(f"    const double* sf = cg_gfnu + {rec['str_base']} + lg_ * {steps};")
+    # the stream in blocks of 4 uint4 (16 steps), each loaded one block ahead of its use.
+    # Memory counters complete in order: block 0 is issued before the writers' old values, so
+    # that the first steps do not wait for those (scheduling barriers keep the order, and keep
+    # the scheduler from hoisting every load to the top, which spills)
+    order = _gather_order(lens)
+    nb = steps // 4
+    blk = 4
+    def load_block(b0):
+        return [f"    const uint4 v{b} = st[{b}];" for b in range(b0, min(nb, b0 + blk))]
+    L += load_block(0)
+    L.append("    __builtin_amdgcn_sched_barrier(0);")
+    # one row pointer per slot for the old value and the store (row 0 for no writer: the load
+    # stays in bounds and the store is skipped); k ld as a 32 x 32 -> 64 product (ld < 2^32)
+    L += [f"    double* const w{s_} = cg_gwc + (uint64_t)(unsigned)(k{s_} >= 0 ? (k{s_} & 0xffffff) : 0) * cg_gld;"
+          for s_ in range(nsl)]
+    # the old values, for the slots with a writer that adds (none in a chunk where every
+    # writer stores: the first chunk)
+    adds = rec["adds"]
+    L += [f"    const double p{s_} = *w{s_};" if adds[s_] else f"    const double p{s_} = 0.0;" for s_ in range(nsl)]
+    L += [f"    double a{s_} = 0.0;" for s_ in range(nsl)]
+    last = {}
+    for t, (s_, u) in enumerate(order):
+        last[s_] = t
+    for b0 in range(0, nb, blk):
+        L.append("    __builtin_amdgcn_sched_barrier(0);")
+        L += load_block(b0 + blk)
+        for t in range(4 * b0, min(len(order), 4 * (b0 + blk))):
+            s_ = order[t][0]
+            wd = f"v{t // 4}.{'xyzw'[t % 4]}"
+            nu = f"sf[{t}]" if frac else f"CG_GNU({wd})"
+            L.append(f"    a{s_} = __builtin_fma({nu}, CG_GQ({wd}), a{s_});")
+            if last[s_] == t:
+                L.append(f"    if (cm & {1 << s_}) {{")
+                L.append(f"      for (int o = CG_NPT; o < 64; o <<= 1) a{s_} += __shfl_xor(a{s_}, o);")
+                L.append("    }")
+                L.append(f"    if (k{s_} >= 0 && valid) *w{s_} = (k{s_} & (1 << 30)) ? a{s_} : p{s_} + a{s_};")
+    L.append("  }")
+    return "\n".join(L) + "\n"
+
+
+def table_gather_plan(mech: Mechanism, npt: int, W: int) -> Dict:
+    """How the table kernel of ``npt``-point tiles and ``W`` wavefronts runs its reactions:
+    the table layout (a class gets a compiled loop when it fills at least half the reaction
+    slots of a tile; JetSurf-sized: 2.6% faster than a full round of slots, every class
+    compiled 5% slower), the chunk size rc (the rates of progress of rc reactions fill the LDS
+    left after EG and h [ns + 1][npt] and the per-wavefront partial sums [W][npt]: every table
+    shape runs one workgroup per CU, W wavefronts = 4 SIMDs x its waves per SIMD), the chunks
+    and their species incidence lists, and the LDS bytes of a tile."""
+    ns, nr = mech.n_species, mech.n_reactions
+    slots = W * 64 // npt
+    fixed = 2 * (ns + 1) * npt + W * npt + npt       # ... and the gather's zero row
+    rc = min(nr, (LDS_PER_CU // 8 - fixed) // npt)
+    if rc < min(nr, slots):
+        raise ValueError(f"table kernel: {npt}-point tiles of {ns} species leave LDS for {rc} rates of progress")
+    lay = table_layout(mech, npt, min_class=max(1, slots // 2))
+    spans = [(a, b) for (a, b, _) in lay.segs] + ([(lay.gen_start, nr)] if lay.gen_start < nr else [])
+    chunks = table_chunks(spans, rc, slots) if nr else []
+    goff, grow, gnu = table_incidence(mech, lay.order, chunks, npt)
+    streams = table_gather_streams(mech, chunks, goff, grow, gnu, npt, W, rc) if chunks else None
+    return dict(layout=lay, rc=rc, slots=slots, spans=spans, chunks=chunks, goff=goff, grow=grow, gnu=gnu,
+                streams=streams, lds=8 * (fixed + rc * npt))
+
+
 def table_records(mech: Mechanism, npt: int = 16, min_class: int = 0):
     """(reaction table rows [2 nr][4], efficiency species, order, class segments, start of the
     generic segment) of ``table_layout``."""
@@ -1099,12 +1362,9 @@ def _emit_table(mech: Mechanism, variant: Variant):
     if 64 % npt:
         raise ValueError("table kernel: points per tile must divide 64")
     slots = W * 64 // npt
-    lds = 8 * (3 * (mech.n_species + 1) * npt + W * npt)
-    if lds > LDS_PER_CU:
-        raise ValueError(f"table kernel: {npt}-point tiles of {mech.n_species} species need {lds} B of LDS")
-    # a class gets a compiled loop when it fills at least half the reaction slots of a tile
-    # (JetSurf-sized: 2.6% faster than a full round of slots, every class compiled 5% slower)
-    lay = table_layout(mech, npt, min_class=max(1, slots // 2))
+    nr = mech.n_reactions
+    plan = table_gather_plan(mech, npt, W)
+    lay, rc, lds = plan["layout"], plan["rc"], plan["lds"]
     rows, effsp, segs, gen_start = lay.rows, lay.effsp, lay.segs, lay.gen_start
     classes = []
     for (a, b, (fam, rev, nrs, nps, dnu, _neg)) in segs:
@@ -1114,6 +1374,12 @@ def _emit_table(mech: Mechanism, variant: Variant):
         classes.append(dict(start=a, end=b, fam=fam, rev=rev, nrs=nrs, nps=nps, dnu=dnu, pf=pf,
                             name=f"family {fam}, {'reversible' if rev else 'irreversible'}, "
                                  f"{nrs} -> {nps} slots, dnu {dnu}"))
+    chunk_list, gs = plan["chunks"], plan["streams"]
+    gfrac = bool(gs and gs["fnu"])
+    chunks = [dict(c0=c0, c1=c1, gather=_gather_code(ci, gs["chunks"][ci], gfrac),
+                   pieces=[dict(a=a, b=b, cl=classes[si] if si < len(classes) else None)
+                                         for (a, b, si) in pieces])
+              for ci, (c0, c1, pieces) in enumerate(chunk_list)]
     env = jinja2.Environment(loader=jinja2.FileSystemLoader(TEMPLATE_DIR), keep_trailing_newline=True)
     tpl = env.get_template("getrates_table_hip.j2")
     h = structure_hash(mech)
@@ -1125,7 +1391,15 @@ def _emit_table(mech: Mechanism, variant: Variant):
         lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
         log10_2=dlit(LOG10_2), log2_10=dlit(LOG2_10), exp_impl=variant.exp_impl, poly=variant.poly,
         rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
-        classes=classes, gen_start=gen_start,
+        classes=classes, gen_start=gen_start, chunks=chunks, rc=rc,
+        gG=gs["G"] if gs else 1,
+        gspk=", ".join(str(v) for v in gs["spk"]) if gs and gs["spk"] else "-1",
+        n_gspk=max(1, len(gs["spk"])) if gs else 1,
+        gcomb=", ".join(str(v) for v in gs["comb"]) if gs and gs["comb"] else "0",
+        n_gcomb=max(1, len(gs["comb"])) if gs else 1,
+        gstr=", ".join(f"{v}u" for v in gs["stream"]) if gs and gs["stream"] else "0u",
+        n_gstr=max(4, len(gs["stream"])) if gs else 4,
+        gfrac=gfrac, gfnu=", ".join(dlit(v) for v in gs["fnu"]) if gfrac else "0.0",
         has_plog=bool(lay.plog), plog=", ".join(f"{{{o}, {n}}}" for o, n in lay.plog), n_plog=len(lay.plog),
         has_long=len(lay.xsl) > 2, xsl=", ".join(str(v) for v in lay.xsl), n_xsl=len(lay.xsl),
         has_neg=any(negative_a(r) for r in mech.reactions),
@@ -1135,7 +1409,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
     meta = dict(ns=mech.n_species, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads,
                 ppb=variant.ppb, waves_per_eu=variant.waves_per_eu, structure_hash=h, slots=slots,
                 compiled_classes=len(classes), generic_reactions=mech.n_reactions - gen_start,
-                lds_bytes=lds)
+                lds_bytes=lds, chunks=len(chunks), chunk_reactions=rc)
     return src, coef, meta
 
 

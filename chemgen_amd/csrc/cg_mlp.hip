@@ -288,28 +288,35 @@ compact_k(const int* __restrict__ flag, int64_t ng, int audit_every, int* __rest
     if (mask & (1u << i)) idx[q++] = b0 + (int64_t)i * 256 + threadIdx.x;
 }
 
-// The surrogate audit's verdict: one thread per audited point, off = some species with
-// |Ws - Wr| / (|Wr| + s_k) > err_tol (or not finite); the integer count is order-independent,
-// so the verdict is deterministic.  The last workgroup to finish (a ticket in acc[1]) sets
-// *n_bad and *reject = (*n_bad > max_bad * n), zeroes *count_clear (the compaction's counter,
-// when given) and clears acc for the next call.
+// The surrogate audit's verdict: a workgroup takes 64 audited points, four lanes per point
+// (species k = g, g + 4, ...: four loads in flight per lane instead of one long serial chain);
+// off = some species with |Ws - Wr| / (|Wr| + s_k) > err_tol (or not finite).  The integer count
+// is order-independent, so the verdict is deterministic.  The last workgroup to finish (a ticket
+// in acc[1]) sets *n_bad and *reject = (*n_bad > max_bad * n), zeroes *count_clear (the
+// compaction's counter, when given) and clears acc for the next call.
 __global__ void __launch_bounds__(256)
 audit_k(const double* __restrict__ Ws, int64_t lds, const double* __restrict__ Wr, int64_t ldr, int64_t n,
         int ns, const double* __restrict__ s, double err_tol, double max_bad, int* __restrict__ reject,
         int* __restrict__ n_bad, unsigned* __restrict__ acc, int* __restrict__ count_clear) {
+  __shared__ int off_pt[64];
   __shared__ int red;
+  const int p = threadIdx.x & 63, g = threadIdx.x >> 6;
+  if (threadIdx.x < 64) off_pt[threadIdx.x] = 0;
   if (threadIdx.x == 0) red = 0;
   __syncthreads();
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t m = (int64_t)blockIdx.x * 64 + p;
   if (m < n) {
     bool off = false;
-    for (int k = 0; k < ns; ++k) {
+#pragma unroll 4
+    for (int k = g; k < ns; k += 4) {
       const double wr = Wr[(int64_t)k * ldr + m];
       const double q = fabs(Ws[(int64_t)k * lds + m] - wr) / (fabs(wr) + s[k]);
       off |= !(q <= err_tol);
     }
-    if (off) atomicAdd(&red, 1);
+    if (off) off_pt[p] = 1;
   }
+  __syncthreads();
+  if (threadIdx.x < 64 && off_pt[threadIdx.x]) atomicAdd(&red, 1);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (red) atomicAdd(&acc[0], (unsigned)red);
@@ -859,7 +866,7 @@ extern "C" int cg_mlp_audit(const double* Ws, int64_t lds, const double* Wr, int
   if (n <= 0 || ns <= 0 || lds < n || ldr < n || !reject || !n_bad || !acc || !Ws || !Wr || !s)
     return cg_internal_fail(CG_EINVAL, "cg_mlp_audit: bad argument");
   if (n > 0x7fffffffLL) return cg_internal_fail(CG_ERANGE, "cg_mlp_audit: n too large");
-  audit_k<<<nblk(n, 256), 256, 0, (hipStream_t)stream>>>(Ws, lds, Wr, ldr, n, ns, s, err_tol, max_bad, reject, n_bad,
+  audit_k<<<nblk(n, 64), 256, 0, (hipStream_t)stream>>>(Ws, lds, Wr, ldr, n, ns, s, err_tol, max_bad, reject, n_bad,
                                                          acc, count_clear);
   CG_LAUNCH_CHECK("cg_mlp_audit launch");
   return CG_OK;

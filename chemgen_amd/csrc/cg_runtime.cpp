@@ -194,6 +194,7 @@ extern "C" int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_
   if (!h) return fail(CG_EINVAL, "cg_getrates: null handle");
   if (ng < 0 || ld < ng) return fail(CG_EINVAL, "cg_getrates: need 0 <= ng <= ld (ng=%lld ld=%lld)",
                                      (long long)ng, (long long)ld);
+  if (ld >= (1LL << 32)) return fail(CG_ERANGE, "cg_getrates: ld must be below 2^32");
   if (ng == 0) return CG_OK;
   if (!P || !T || !Y || !W) return fail(CG_EINVAL, "cg_getrates: null array");
   const int64_t nb = (ng + h->meta.ppb - 1) / h->meta.ppb;
@@ -288,6 +289,7 @@ static int tiles_launch(cg_mech* h, const int64_t* tile_start, const int32_t* ti
   if (ld < n_points)
     return fail(CG_EINVAL, "%s: ld = %lld < %lld points of the batch", who, (long long)ld,
                 (long long)n_points);
+  if (ld >= (1LL << 32)) return fail(CG_ERANGE, "%s: ld must be below 2^32", who);
   // grid > 0: that many persistent workgroups; 0: 8 per CU; < 0: one workgroup per tile
   int64_t g = grid > 0 ? grid : (grid < 0 ? nt : (int64_t)h->n_cu * 8);
   if (g > nt) g = nt;

@@ -88,10 +88,10 @@ def variant_for(args, mech):
     if lvl == 2:
         return Variant(64, 2, "lean", "gh", "horner", False, 8, max(2, args.nreact_per_block))
     if lvl == 3:
-        from .autotune import LDS_BYTES, table_lds_bytes
+        from .autotune import table_fits
         npt = min((1, 2, 4, 8, 16, 32, 64), key=lambda n: abs(n - args.veclen))
         waves = max(1, args.nreact_per_block)
-        if table_lds_bytes(mech.n_species, npt, waves) > LDS_BYTES:
+        if not table_fits(mech.n_species, mech.n_reactions, npt, waves):
             raise SystemExit(f"--parallel-level 3: {npt} points x {mech.n_species} species exceed the LDS; "
                              "use a smaller --veclen")
         return Variant(64, max(1, min(4, waves // 4)), "lean", "gh", "horner", False, 4, waves, npt)

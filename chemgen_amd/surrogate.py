@@ -435,7 +435,7 @@ class Surrogate:
         # packed recompute of the flagged points, on a bounded grid (the tile kernel strides over
         # the list, so an empty list costs one short launch)
         Pp, Tp, Yp = self._gather(P, T, Y, ws["idx"], ws["pcount"], ng, out=ws["packed"], stream=cur)
-        fallback.getrates_tiles_(Pp, Tp, Yp, ws["Wp"], ws["ts"], ws["tl_p"], ng, grid=1024, stream=cur)
+        fallback.getrates_tiles_(Pp, Tp, Yp, ws["Wp"], ws["ts"], ws["tl_p"], ng, grid=512, stream=cur)
         self._scatter(ws["Wp"], ws["idx"], ws["pcount"], ng, W, stream=cur)
         # the whole batch in FP64 (zero-length tiles unless the plan says so)
         fallback.getrates_tiles_(P, T, Y, W, ws["ts"], ws["tl_f"], ng, grid=-1, stream=cur)

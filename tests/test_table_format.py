@@ -162,3 +162,112 @@ def test_generator_level3_short_tiles():
     assert variant_for(Namespace(parallel_level=3, veclen=1, nreact_per_block=4), m).table == 1
     with pytest.raises(SystemExit):
         variant_for(Namespace(parallel_level=3, veclen=16, nreact_per_block=16), m)
+
+
+@pytest.mark.parametrize("name,npt,W", [("synth_jetsurf348", 16, 16), ("fuller2021", 8, 16), ("hashemi2016", 16, 16),
+                                        ("synth_ext53", 16, 4), ("gri30", 64, 8), ("synth_plog53", 2, 16)])
+def test_chunks_and_incidence_give_the_stoichiometry(name, npt, W):
+    """The table kernel's chunks cover the table in order, each within the LDS buffer, as
+    few as the buffer allows (ceil(reactions / rc)); and its per-chunk species lists, applied to rates of
+    progress, give wdot_k = sum_j (nu''_kj - nu'_kj) q_j of the mechanism's own coefficients
+    (a dense product over the reactions in mechanism order)."""
+    import numpy as np
+    from chemgen_amd.codegen import table_gather_plan
+    m = load_bundled(name)
+    plan = table_gather_plan(m, npt, W)
+    lay, rc, slots, spans, chunks = plan["layout"], plan["rc"], plan["slots"], plan["spans"], plan["chunks"]
+    assert chunks[0][0] == 0 and chunks[-1][1] == m.n_reactions
+    assert len(chunks) == -(-m.n_reactions // rc)
+    for (c0, c1, pieces), nxt in zip(chunks, chunks[1:] + [(m.n_reactions, None, None)]):
+        assert c1 == nxt[0] and 0 < c1 - c0 <= rc
+        assert pieces[0][0] == c0 and pieces[-1][1] == c1
+        for (a, b, si), after in zip(pieces, pieces[1:] + [(c1, None, None)]):
+            sa, sb = spans[si]
+            assert sa <= a < b <= sb and b == after[0]
+    assert plan["lds"] <= 160 * 1024
+    # the lists applied to random rates of progress
+    rng = np.random.default_rng(0)
+    q = rng.normal(size=(m.n_reactions, npt))             # in table order
+    w = np.zeros((m.n_species, npt))
+    goff, grow, gnu = plan["goff"], plan["grow"], plan["gnu"]
+    for c, (c0, c1, _) in enumerate(chunks):
+        buf = q[c0:c1].reshape(-1)                          # the chunk's LDS buffer [row][pt]
+        for k in range(m.n_species):
+            for e in range(goff[c * (m.n_species + 1) + k], goff[c * (m.n_species + 1) + k + 1]):
+                assert grow[e] % npt == 0 and grow[e] < (c1 - c0) * npt
+                w[k] += gnu[e] * buf[grow[e]:grow[e] + npt]
+    N = _net_matrix(m)
+    q_mech = np.empty_like(q)
+    q_mech[lay.order] = q                                   # table position n holds reaction order[n]
+    assert np.allclose(w, N @ q_mech, rtol=1e-12, atol=1e-12)
+    # the kernel's gather loop, run on the host-built schedule
+    wk = _emulate_gather(plan, m.n_species, npt, W, q)
+    assert np.allclose(wk, N @ q_mech, rtol=1e-12, atol=1e-12)
+
+
+def _net_matrix(m):
+    import numpy as np
+    idx = {s.name: i for i, s in enumerate(m.species)}
+    N = np.zeros((m.n_species, m.n_reactions))
+    for j, r in enumerate(m.reactions):
+        for sp, nu in r.reactants.items():
+            N[idx[sp], j] -= nu
+        for sp, nu in r.products.items():
+            N[idx[sp], j] += nu
+    return N
+
+
+def _emulate_gather(plan, ns, npt, W, q):
+    """The table kernel's gather on the CPU, as its generated code runs it: in chunk c every
+    lane group (w, g) sums its stream slot by slot (common slot lengths), a split slot's parts
+    are summed over the wavefront's groups, and each slot's writer stores (first chunk) or
+    adds.  Every species must be stored exactly once, before any add."""
+    import struct
+    import numpy as np
+    from chemgen_amd.codegen import _gather_order
+    st = plan["streams"]
+    G, rc = st["G"], plan["rc"]
+    assert G * npt == 64
+    out = np.full((ns, npt), np.nan)
+    stored = set()
+    for c, (c0, c1, _) in enumerate(plan["chunks"]):
+        rec = st["chunks"][c]
+        lens, steps = rec["lens"], rec["steps"]
+        assert steps % 4 == 0 and steps >= sum(lens)
+        buf = np.zeros((rc + 1) * npt)
+        buf[:(c1 - c0) * npt] = q[c0:c1].reshape(-1)
+        for w in range(W):
+            cm = st["comb"][rec["comb_base"] + w]
+            a = np.zeros((len(lens), G, npt))
+            order = _gather_order(lens)
+            for g in range(G):
+                base = rec["str_base"] + (w * G + g) * steps
+                for t, (s_, _) in enumerate(order):
+                    if True:
+                        e = st["stream"][base + t]
+                        o = (e & 0x3ffff) // 8
+                        assert (e & 0x3ffff) % (8 * npt) == 0 and o <= rc * npt
+                        if st["fnu"] is not None:
+                            nu = st["fnu"][base + t]
+                        else:                                    # the double {0, bits 18-31}
+                            nu = struct.unpack("<d", struct.pack("<II", 0, e & 0xfffc0000))[0]
+                        a[s_, g] = nu * buf[o:o + npt] + a[s_, g]
+                for e in st["stream"][base + len(order):base + steps]:    # padding after the last slot
+                    assert e == rc * npt * 8
+            for s_ in range(len(lens)):
+                if cm & (1 << s_):
+                    a[s_, :] = a[s_].sum(axis=0)
+                for g in range(G):
+                    k = st["spk"][rec["spk_base"] + (w * G + g) * len(lens) + s_]
+                    if k < 0:
+                        continue
+                    sp = k & 0xffffff
+                    if k & (1 << 30):
+                        assert sp not in stored
+                        stored.add(sp)
+                        out[sp] = a[s_, g]
+                    else:
+                        assert sp in stored
+                        out[sp] = out[sp] + a[s_, g]
+    assert stored == set(range(ns))
+    return out
