@@ -1,9 +1,10 @@
 """Small drivers to run one kernel repeatedly under a profiler (rocprofv3 --kernel-trace /
---pmc): the FP64 getrates kernel of a bundled mechanism, the fused surrogate inference, or
-the ragged-batch launch.
+--pmc): the FP64 getrates kernel of a bundled mechanism, the fused surrogate inference, the
+whole surrogate path (audit, network, FP64 fallback), or the ragged-batch launch.
 
     python -m chemgen_amd.profile_run getrates --mech synth_gri53 --ng 1048576 --reps 5
     python -m chemgen_amd.profile_run infer --mech synth_gri53 --ng 1048576 --reps 5
+    python -m chemgen_amd.profile_run surpath --mech synth_gri53 --ng 1048576 --reps 5
     python -m chemgen_amd.profile_run batched --mech synth_nhept88 --reps 5
     python -m chemgen_amd.profile_run gemm --mech fuller2021 --ng 65536 --reps 5
     python -m chemgen_amd.profile_run gemm --shape 3761 16532 --ng 16384 --reps 5
@@ -46,7 +47,7 @@ def main(argv=None):
     from .runtime import DeviceMechanism
     from .surrogate import Surrogate, random_states
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["getrates", "infer", "batched", "gemm", "host"])
+    ap.add_argument("what", choices=["getrates", "infer", "surpath", "batched", "gemm", "host"])
     ap.add_argument("--mech", default="synth_gri53")
     ap.add_argument("--ng", type=int, default=1 << 20)
     ap.add_argument("--reps", type=int, default=5)
@@ -92,10 +93,17 @@ def main(argv=None):
         sur.load_state_dict(d)
         P, T, Y = random_states(m.n_species, a.ng, seed=7, T_range=(a.tlo, a.thi))
         W = torch.empty_like(Y)
-        _warm(lambda: sur.predict_(P, T, Y, W, fused=True), a.warm)
-        for _ in range(a.reps):
-            sur.predict_(P, T, Y, W, fused=True)
-        tag = f"MLP {dims}"
+        if a.what == "surpath":     # the whole surrogate path: audit, network, FP64 fallback
+            dm = DeviceMechanism(m, best_variant(m), build=False)
+            _warm(lambda: sur.getrates_(P, T, Y, W, dm), a.warm)
+            for _ in range(a.reps):
+                info = sur.getrates_(P, T, Y, W, dm)
+            tag = f"MLP {dims} path {Surrogate.fallback_info(info)}"
+        else:
+            _warm(lambda: sur.predict_(P, T, Y, W, fused=True), a.warm)
+            for _ in range(a.reps):
+                sur.predict_(P, T, Y, W, fused=True)
+            tag = f"MLP {dims}"
     torch.cuda.synchronize()
     print(f"done {a.what} {a.mech} {tag}", flush=True)
 
