@@ -108,8 +108,8 @@ def cpu_baseline(mech_name, mech, P, T, Y, target_s=15.0, sample_ng=1 << 14, hos
     ``sample_ng`` of the GPU's own (seed 0) states: all OpenMP threads (OMP_PROC_BIND=close,
     OMP_PLACES=cores) and one thread, median of 5 runs each, for the strict build (libm) and
     the vectorised build (libmvec; AVX-512 where the CPU has it); ``value`` is the faster
-    all-thread figure.  Also the configs[0] Python-language reference: the NumPy oracle on the
-    H2 Burke mechanisms at ng = 4096, one thread."""
+    all-thread figure.  Also the reference's Python-language getrates as restated in NumPy by
+    the oracle, on the H2 Burke mechanisms at ng = 4096, one thread."""
     import numpy as np
     from chemgen_amd.kernels import load_bundled
     from chemgen_amd.synth import make_states
@@ -134,7 +134,8 @@ def cpu_baseline(mech_name, mech, P, T, Y, target_s=15.0, sample_ng=1 << 14, hos
                                       "one_thread_points_per_s": one_t, "one_thread_runs": runs_one,
                                       "parity_max_gross": par["max_gross"]}
     best = max(builds, key=lambda k: builds[k]["points_per_s"])
-    # the Python-language reference (NumPy oracle), one thread, ng = 4096
+    # the reference's Python-language getrates as the NumPy restatement of the oracle times it,
+    # one thread, 4096 points (the reference's own Python is not run)
     py = {}
     for h2 in ("h2_burke2012", "h2_burke"):
         m0 = load_bundled(h2)
@@ -147,7 +148,7 @@ def cpu_baseline(mech_name, mech, P, T, Y, target_s=15.0, sample_ng=1 << 14, hos
                       f"seed-0 states of {mech_name}; C/OpenMP restatement of the reference's getrates_i "
                       f"({best} build)",
             "build": best, "builds": builds, "host": host or host_info(),
-            "python_reference_ng4096": py}
+            "numpy_restatement_ng4096": py}
 
 
 def bench_batched(steps, warmup):

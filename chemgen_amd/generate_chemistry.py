@@ -15,7 +15,9 @@ kernels read (the layout is documented in chemgen_amd/codegen.py); with ``--comp
 reference`` also DIR/coef_m_vN.f90 (N = --parallel-level), the reference's Fortran
 coefficient module that its ``chemgen_m`` uses (chemgen_amd/legacy_coef.py): species maps at
 levels 1-3 (MAX_SP a power of two at 3), dense rocBLAS matrices at 4, each reaction type padded
-to a multiple of --nreact-per-block with the reference's dummy reactions.
+to a multiple of --nreact-per-block with the reference's dummy reactions (at every level, as the
+reference's gpu_coef mode does; with --compat reference the defaults of --parallel-level,
+--veclen and --nreact-per-block are the reference's: 1, 16, 8).
 
 Launch shape, the analogue of the reference's parallel levels (--parallel-level,
 --veclen, --nreact-per-block):
@@ -51,9 +53,13 @@ def parse_arguments(argv=None):
     p.add_argument("--therm")
     p.add_argument("--yaml_file")
     p.add_argument("--output", required=True)
-    p.add_argument("--parallel-level", type=int, choices=[1, 2, 3, 4], default=4)
-    p.add_argument("--veclen", type=int, default=64)
-    p.add_argument("--nreact-per-block", type=int, default=4)
+    # defaults: this build's (the autotuned shape) unless --compat reference, which takes the
+    # reference's (generate_chemistry.py:21-26: level 1, VECLEN 16, 8 reactions per block), so
+    # that a coefficient module written without these flags has the layout the reference's own
+    # would have
+    p.add_argument("--parallel-level", type=int, choices=[1, 2, 3, 4], default=None)
+    p.add_argument("--veclen", type=int, default=None)
+    p.add_argument("--nreact-per-block", type=int, default=None)
     p.add_argument("--language", choices=["python", "fortran", "hip"], default="hip")
     p.add_argument("--omp", action="store_true")
     p.add_argument("--vector", action="store_true")
@@ -74,7 +80,12 @@ def parse_arguments(argv=None):
     p.add_argument("--legacy-abi", action="store_true",
                    help="also build lib<stem>_legacy.so exporting the reference's C entry points "
                         "(copyConstantsToDevice, getrates_hip_wrapper_v1..v4) over this kernel")
-    return p.parse_args(argv)
+    a = p.parse_args(argv)
+    ref = a.compat == "reference"
+    for name, ours, theirs in (("parallel_level", 4, 1), ("veclen", 64, 16), ("nreact_per_block", 4, 8)):
+        if getattr(a, name) is None:
+            setattr(a, name, theirs if ref else ours)
+    return a
 
 
 def variant_for(args, mech):

@@ -55,11 +55,17 @@ def test_generator_writes_the_module(tmp_path):
     import os
     from chemgen_amd.generate_chemistry import main
     from chemgen_amd.kernels import DATA_DIR
+    # with --compat reference the defaults are the reference's (generate_chemistry.py:21-26):
+    # level 1, and each reaction type padded to a multiple of 8 -- at level 1 too, as the
+    # reference's gpu_coef mode passes nreact_per_block to write_coef_module at every level
+    main(["--mode", "gpu_coef", "--mech", os.path.join(DATA_DIR, "h2_burke.inp"), "--output", str(tmp_path),
+          "--compat", "reference"])
+    src = (tmp_path / "coef_m_v1.f90").read_text()
+    assert parse_module(src) == parse_module(emit_coef_module(load_bundled("h2_burke"), 1, 8))
     main(["--mode", "gpu_coef", "--mech", os.path.join(DATA_DIR, "h2_burke.inp"), "--output", str(tmp_path),
           "--compat", "reference", "--parallel-level", "2"])
     src = (tmp_path / "coef_m_v2.f90").read_text()
-    # the CLI pads each reaction type to a multiple of --nreact-per-block (default 4)
-    assert parse_module(src) == parse_module(emit_coef_module(load_bundled("h2_burke"), 2, 4))
+    assert parse_module(src) == parse_module(emit_coef_module(load_bundled("h2_burke"), 2, 8))
     for level in (1, 3, 4):
         main(["--mode", "gpu_coef", "--mech", os.path.join(DATA_DIR, "h2_burke.inp"), "--output", str(tmp_path),
               "--compat", "reference", "--parallel-level", str(level), "--nreact-per-block", "8"])

@@ -115,11 +115,12 @@ _LIBS = {1: "liblegacy_h2_burke.so", 2: "liblegacy_h2_burke.so", 3: "liblegacy_h
          4: "liblegacy_h2_burke_v4.so"}
 
 
-def level_args(mech, level, rpb=8):
+def level_args(mech, level, rpb=8, pad_level_1=False):
     """copyConstantsToDevice's arguments, in order, from this build's coefficient module of the
-    level (what chemgen_m passes)."""
+    level (what chemgen_m passes); level 1 unpadded unless ``pad_level_1`` (the reference's
+    gpu_coef mode pads it too, with its default of 8 reactions per block)."""
     from chemgen_amd.legacy_abi import _params, _params4
-    arrs = parse_coef_module(emit_coef_module(mech, level, rpb if level > 1 else None))
+    arrs = parse_coef_module(emit_coef_module(mech, level, rpb if level > 1 or pad_level_1 else None))
     present = [t for t in ("standard", "troe", "third_body") if any(r.rtype == t for r in mech.reactions)]
     names = [p.split()[-1] for p in (_params(present) if level < 4 else _params4(present))]
     return [np.ascontiguousarray(arrs[n]) for n in names]
@@ -147,6 +148,23 @@ def test_converter_reads_each_level(level):
     scale = np.maximum(np.abs(ref), 1.0)
     assert np.all(np.isfinite(cf) | np.isinf(ref))
     assert np.max(np.abs(np.where(np.isinf(ref), 0.0, cf - ref)) / scale) < 1e-13
+
+
+def test_converter_reads_padded_level_1():
+    """The reference's default coefficient module (level 1, each type padded to 8 with dummy
+    reactions) converts to the same blob as the unpadded one."""
+    from chemgen_amd.codegen import build_coef
+    L = load_level_library(1)
+    m = load_bundled("h2_burke")
+    ref, _, _ = build_coef(m)
+    outs = []
+    for pad in (False, True):
+        args = level_args(m, 1, pad_level_1=pad)
+        cf = np.full(len(ref), np.nan)
+        assert L.cg_legacy_convert(*[ctypes.c_void_p(a.ctypes.data) for a in args],
+                                   ctypes.c_void_p(cf.ctypes.data)) == 0
+        outs.append(cf)
+    assert np.array_equal(outs[0], outs[1], equal_nan=True)
 
 
 def test_converter_refuses_other_level():
