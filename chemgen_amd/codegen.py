@@ -72,7 +72,7 @@ class Variant:
     kr_mode: str = "gh"      # "gh": q = kf prod(g_r) (prod(h_r) - pfac^-dnu prod(h_p)) with
                              #   g = exp(smh), h = C/g per species (no per-reaction division);
                              # "eg": kb from g products and one division; "exp": one exp per kb
-    poly: str = "horner"     # exp polynomial evaluation: "horner" or "estrin" (shorter chains)
+    poly: str = "horner"     # exp2 polynomial: "horner" (degree 10), "estrin" (shorter chains), "d9" (degree 9, 1.5e-14)
     g_lds: bool = False      # keep the per-species EG_k in LDS instead of registers
     group: int = 4           # with LDS state: reactions between re-reads from LDS
     split: int = 1           # wavefronts per 64-point tile sharing the species state in LDS
