@@ -87,12 +87,16 @@ class Variant:
     pair: bool = False       # straight-line kernels: the exp2 of two consecutive Arrhenius rates
                              # (and of two species' EG) evaluated interleaved, so that each
                              # wavefront has two independent FMA chains in flight
+    fdiv: bool = False       # divisions inside log2, h_k = C_k / EG_k and the falloff blending by
+                             # a reciprocal with one Newton step (cg_div_fast, a few ulp)
 
     @property
     def tag(self) -> str:
         if self.table:
-            return f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
-        c = ("k" if self.baked else "") + ("o" if self.live else "") + ("p" if self.pair else "")
+            return (f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
+                    + ("d" if self.fdiv else ""))
+        c = (("k" if self.baked else "") + ("o" if self.live else "") + ("p" if self.pair else "")
+             + ("d" if self.fdiv else ""))
         if self.split > 1:
             return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
                     f"{self.poly[0]}g{self.group}{c}")
@@ -338,12 +342,14 @@ def _plog_code(r: Reaction, off: int) -> List[str]:
 
 
 def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: str = "exp",
-                   skip_eff=frozenset(), qss=None, kf_var: str = None) -> List[str]:
+                   skip_eff=frozenset(), qss=None, kf_var: str = None, fdiv: bool = False) -> List[str]:
     """Straight-line code of one reaction: kf (and M), q and the wdot updates.  ``skip_eff``:
     species left out of the third-body sum.  ``qss`` = (QSS species -> unknown index, stage):
     stage 1 emits the reaction's terms of the QSS balance, stage 2 its rate with the solved
     QSS concentrations x<i> (see chemgen_amd/qssa.py).  ``kf_var``: an elementary or
-    third-body reaction's 2^(log2 k) was computed before (``_paired_code``) into this variable."""
+    third-body reaction's 2^(log2 k) was computed before (``_paired_code``) into this variable.
+    ``fdiv``: the falloff divisions use the one-Newton-step reciprocal."""
+    div = "cg_div_fast" if fdiv else "cg_div"
     L = [f"  {{  // R{j}: {_eq(r)}" + (f" [QSS stage {qss[1]}]" if qss else "")]
     if kr_mode == "gh":
         cvar = lambda k: f"(g{k} * h{k})"
@@ -416,19 +422,19 @@ def _reaction_code(j: int, r: Reaction, off: int, idx: Dict[str, int], kr_mode: 
             # log10 F = log10 Fcent / (1 + (x/d)^2) = log10 Fcent d^2 / (d^2 + x^2): one division
             L.append("    const double x = lp + cc, d = nn - 0.14 * x;")
             # k_inf F = 2^(log2 k_inf + log2 F): one exp2 for both
-            L.append("    const double l2F = cg_div(l2f * (d * d), __builtin_fma(x, x, d * d));")
-            L.append("    const double kf = cg_exp2(lki + l2F) * cg_div(pr, 1.0 + pr);")
+            L.append(f"    const double l2F = {div}(l2f * (d * d), __builtin_fma(x, x, d * d));")
+            L.append(f"    const double kf = cg_exp2(lki + l2F) * {div}(pr, 1.0 + pr);")
         elif r.rtype == "sri":
             L.append(f"    const double lp = cg_log2(pr) * {dlit(LOG10_2)};")
-            L.append("    const double X = cg_div(1.0, 1.0 + lp * lp);")
+            L.append(f"    const double X = {div}(1.0, 1.0 + lp * lp);")
             L.append(f"    const double base = cf[{o}] * cg_exp2(-cf[{o + 1}] * invT) + cg_exp2(-t * cf[{o + 2}]);")
             # k_inf F = d 2^(log2 k_inf + X log2 base [+ e log2 T]): one exp2 for all three
             l2F = "X * cg_log2(base)"
             if len(r.sri) == 5:   # compiled for e = 0 too: e is a coefficient, not structure
                 l2F = f"__builtin_fma(cf[{o + 4}], lg2T, {l2F})"
-            L.append(f"    const double kf = cf[{o + 3}] * cg_exp2(lki + {l2F}) * cg_div(pr, 1.0 + pr);")
+            L.append(f"    const double kf = cf[{o + 3}] * cg_exp2(lki + {l2F}) * {div}(pr, 1.0 + pr);")
         else:
-            L.append("    const double kf = kinf * cg_div(pr, 1.0 + pr);")
+            L.append(f"    const double kf = kinf * {div}(pr, 1.0 + pr);")
         if r.reversible and kr_mode != "gh":
             if kr_mode == "eg":
                 L.append(f"    const double kr = {_kr_eg(r, idx, 'kf')};")
@@ -471,7 +477,7 @@ def _pairable(r: Reaction, kr_mode: str) -> bool:
 
 
 def _reactions_code(js: List[int], mech: Mechanism, roffs: List[int], idx: Dict[str, int], kr_mode: str,
-                    pair: bool, indent: str = "") -> List[List[str]]:
+                    pair: bool, indent: str = "", fdiv: bool = False) -> List[List[str]]:
     """Code blocks of the reactions ``js`` in order.  With ``pair``, two consecutive Arrhenius
     rates share one block whose two exp2 evaluations are interleaved (cg_exp2_fin2), then each
     reaction's body follows as before; returns one block per reaction or pair."""
@@ -487,13 +493,13 @@ def _reactions_code(js: List[int], mech: Mechanism, roffs: List[int], idx: Dict[
                  "    double kfp0, kfp1;",
                  f"    cg_exp2_fin2({_lk(roffs[j], r.arrh[1], r.arrh[2])}, "
                  f"{_lk(roffs[j2], r2.arrh[1], r2.arrh[2])}, kfp0, kfp1);"]
-            L += _reaction_code(j, r, roffs[j], idx, kr_mode, kf_var="kfp0")
-            L += _reaction_code(j2, r2, roffs[j2], idx, kr_mode, kf_var="kfp1")
+            L += _reaction_code(j, r, roffs[j], idx, kr_mode, kf_var="kfp0", fdiv=fdiv)
+            L += _reaction_code(j2, r2, roffs[j2], idx, kr_mode, kf_var="kfp1", fdiv=fdiv)
             L.append("  }")
             out.append([indent + ln for ln in L])
             q += 2
         else:
-            out.append([indent + ln for ln in _reaction_code(j, r, roffs[j], idx, kr_mode)])
+            out.append([indent + ln for ln in _reaction_code(j, r, roffs[j], idx, kr_mode, fdiv=fdiv)])
             q += 1
     return out
 
@@ -607,7 +613,7 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
     for j, r in enumerate(mech.reactions):
         if variant.g_lds and j > 0 and j % max(1, variant.group) == 0:
             lines.append("  CG_FENCE();")
-        lines.extend(_reaction_code(j, r, roffs[j], idx, variant.kr_mode))
+        lines.extend(_reaction_code(j, r, roffs[j], idx, variant.kr_mode, fdiv=variant.fdiv))
     tmids = []
     group_of = []
     for s in mech.species:
@@ -629,7 +635,7 @@ def emit_hip(mech: Mechanism, variant: Variant = Variant()):
         exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
         g_lds=variant.g_lds and variant.kr_mode in ("eg", "gh"),
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of,
-        reactions_code="\n".join(lines), threads=variant.threads, ppb=variant.ppb,
+        reactions_code="\n".join(lines), threads=variant.threads, ppb=variant.ppb, fdiv=variant.fdiv,
         **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
@@ -705,10 +711,10 @@ def _emit_live(mech: Mechanism, variant: Variant):
                       "    hl = __builtin_fma(a[5], t4, hl);  hh = __builtin_fma(a[12], t4, hh);",
                       "    hl = hl + a[0];                    hh = hh + a[7];",
                       f"    g{k} = cg_exp2_fin(lo{group_of[k]} ? hl : hh);",
-                      f"    h{k} = cg_div(ck, g{k});",
+                      f"    h{k} = {'cg_div_fast' if variant.fdiv else 'cg_div'}(ck, g{k});",
                       "  }",
                       f"  double w{k} = 0.0;"]
-        lines.extend(_reaction_code(j, mech.reactions[j], roffs[j], idx, "gh"))
+        lines.extend(_reaction_code(j, mech.reactions[j], roffs[j], idx, "gh", fdiv=variant.fdiv))
         for k in sorted(S[j]):
             if last[k] == pos:
                 lines.append(f"  W[{k} * ld + i] = w{k};")
@@ -722,7 +728,7 @@ def _emit_live(mech: Mechanism, variant: Variant):
         R0=dlit(R0), lg2_patm_r0=dlit(math.log2(PATM / R0)),
         patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, poly=variant.poly,
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], program_code="\n".join(lines),
-        threads=variant.threads, ppb=variant.ppb, **_baked_args(coef, variant),
+        threads=variant.threads, ppb=variant.ppb, fdiv=variant.fdiv, **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.block,
                 waves_per_eu=variant.waves_per_eu, structure_hash=h, peak_live=peak)
@@ -881,7 +887,7 @@ def _emit_split(mech: Mechanism, variant: Variant):
             if g0 > 0:
                 lines.append("    CG_FENCE();")
             for blk in _reactions_code(parts[v][g0:g0 + gsz], mech, roffs, idx, variant.kr_mode,
-                                       variant.pair, "  "):
+                                       variant.pair, "  ", fdiv=variant.fdiv):
                 lines.extend(blk)
         for j in parts[v]:
             r = mech.reactions[j]
@@ -905,7 +911,7 @@ def _emit_split(mech: Mechanism, variant: Variant):
         r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of, species_of=species_of,
         zero_of=species_of, touched=touched, reactions_code=codes, compact_lds=compact_lds,
-        pair=variant.pair,
+        pair=variant.pair, fdiv=variant.fdiv,
         **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,
@@ -1390,7 +1396,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
         hash_hex=f"0x{h:016x}", off_invmw=0, off_smh=off_smh, R0=dlit(R0),
         lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
         log10_2=dlit(LOG10_2), log2_10=dlit(LOG2_10), exp_impl=variant.exp_impl, poly=variant.poly,
-        rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
+        fdiv=variant.fdiv, rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
         classes=classes, gen_start=gen_start, chunks=chunks, rc=rc,
         gG=gs["G"] if gs else 1,
         gspk=", ".join(str(v) for v in gs["spk"]) if gs and gs["spk"] else "-1",

@@ -6,6 +6,8 @@ the structure hash records it.  SRI's T^e factor is compiled whenever the SRI re
 parameters (e = 0 included), so e = 0 and e = 0.2 give the same hash and the same source.
 The one-shot API caches loaded mechanisms per (name, structure, blob).
 """
+import dataclasses
+
 from chemgen_amd.api import cache_key
 from chemgen_amd.codegen import Variant, build_coef, emit_hip, structure_hash
 from chemgen_amd.kernels import load_bundled
@@ -37,3 +39,22 @@ def test_cache_key_separates_coefficient_values():
     assert structure_hash(a) == structure_hash(b)
     assert cache_key(a) != cache_key(b)
     assert cache_key(a) == cache_key(_sri_mech(0.2))
+
+
+def test_fdiv_variant_divides_with_one_newton_step():
+    """The ``fdiv`` option (every tuner candidate) divides with the reciprocal estimate and one
+    Newton step in log2, h_k = C_k / EG_k and the falloff blending, in every kernel family; the
+    default variant keeps the corrected quotient; the option is part of the code-object tag."""
+    from chemgen_amd.autotune import candidate_variants
+    from chemgen_amd.codegen import Variant, emit_hip
+    from chemgen_amd.kernels import load_bundled
+    m = load_bundled("gri30")   # Troe (with and without T2) and Lindemann reactions
+    assert all(v.fdiv for v in candidate_variants(m))
+    for v in (Variant(), Variant(64, 2, "lean", "gh", "horner", False, 28, 4),
+              Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True),
+              Variant(64, 4, "lean", "gh", "horner", False, 4, 16, 16)):
+        slow, _, _ = emit_hip(m, v)
+        fast, _, _ = emit_hip(m, dataclasses.replace(v, fdiv=True))
+        assert "cg_div_fast(m - 1.0, m + 1.0)" in fast and "cg_div(m - 1.0, m + 1.0)" in slow
+        assert fast.count("cg_div_fast(") > slow.count("cg_div_fast(") + 3, v.tag
+        assert dataclasses.replace(v, fdiv=True).tag == v.tag + "d"
