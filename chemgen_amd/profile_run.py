@@ -109,7 +109,8 @@ def main(argv=None):
 
 
 def _gemm(a):
-    """The stoichiometric GEMMs of the GEMM formulation against the getrates kernel."""
+    """The stoichiometric GEMMs of the GEMM formulation, and for a bundled mechanism the complete
+    GEMM-formulation path (chemgen_amd.gemm), against the getrates kernel."""
     import json
     import numpy as np
     import torch
@@ -155,14 +156,24 @@ def _gemm(a):
     out = dict(what="gemm", mech=name, ns=ns, nr=nr, ng=a.ng, gemm_ms=ms_gemm,
                gemm_tflops=flops / ms_gemm * 1e-9, gemm_ns_per_point=ms_gemm * 1e6 / a.ng)
     if m is not None:
+        from .gemm import GemmGetRates
         v = best_variant(m)
         dm = DeviceMechanism(m, v, build=False)
         P, T, Y = random_states(ns, a.ng, seed=3, T_range=(a.tlo, a.thi))
         Wk = torch.empty_like(Y)
         _warm(lambda: dm.getrates_(P, T, Y, Wk), a.warm)
         ms_k = _time_ms(lambda: dm.getrates_(P, T, Y, Wk), a.reps)
+        # the complete GEMM-formulation path (cg_getrates_gemm: element-wise kernels + DGEMMs)
+        gp = GemmGetRates(m, slab=min(a.ng, 1 << 16))
+        Wg = torch.empty_like(Y)
+        _warm(lambda: gp.getrates_(P, T, Y, Wg), a.warm)
+        ms_p = _time_ms(lambda: gp.getrates_(P, T, Y, Wg), a.reps)
+        scale = torch.clamp(Wk.abs().amax(dim=0, keepdim=True), min=1e-300)
         out.update(kernel=v.tag, getrates_ms=ms_k, getrates_ns_per_point=ms_k * 1e6 / a.ng,
-                   gemm_over_getrates=ms_gemm / ms_k)
+                   gemm_over_getrates=ms_gemm / ms_k, gemm_path_ms=ms_p,
+                   gemm_path_ns_per_point=ms_p * 1e6 / a.ng, gemm_path_over_getrates=ms_p / ms_k,
+                   gemm_path_vs_kernel_max_diff=float(((Wg - Wk).abs() / scale).max()))
+        gp.close()
     print(json.dumps(out), flush=True)
 
 

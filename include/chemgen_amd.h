@@ -225,6 +225,31 @@ int cg_mlp_infer(const double* P, const double* T, const double* Y, int64_t ld, 
  * flops), one result per lane into `out` (device, blocks * 256 doubles). */
 int cg_probe_fp64_fma(double* out, int32_t blocks, int32_t iters, void* stream);
 
+/* ---- getrates in the GEMM formulation (the reference's parallel level 4) ----
+ * Replaces getrates_hip_wrapper_v4 + allocate_intermediate_DeviceMemory
+ * (chemgen/templates/reactions_hip/hip_v4_h.j2:26-32, hip_v4_cpp.j2:422-597,
+ * chemgen/constants.py:435-470): the stoichiometric sums as FP64 DGEMMs (rocBLAS) with dense
+ * coefficient matrices, element-wise kernels around them.  A plan owns device copies of the
+ * matrices, the blob, a rocBLAS handle and a workspace for `slab` grid points
+ * ((3 Ns + 4 Nr + Ntb + 4) slab doubles); cg_getrates_gemm works through the points slab by
+ * slab on `stream`.  Calls on one plan must not overlap (they share its workspace).
+ *   coef/n_coef/off_smh: the gpu_coef blob and the offset of its NASA-7 block; tmid[Ns]: switch
+ *   temperatures; nu_r, nu_p [Nr][Ns]: reactant and product coefficients; eff [Ntb][Ns]:
+ *   collision efficiencies (1 for unlisted species) of the reactions with a third body;
+ *   rec [Nr][8]: kind (0 elementary, 1 third-body, 2 Lindemann, 3 Troe, 4 Troe with T2, 5 SRI,
+ *   6 SRI with T^e, 7 PLOG), blob offset of the rate, row of eff (-1: none), reversible,
+ *   negative prefactor, first and count of the reaction's pressure points in plog, 0;
+ *   dnu[Nr]: sum nu_p - sum nu_r; plog[n_plog][2]: blob offset of a pressure point's record
+ *   and its number of Arrhenius terms. */
+typedef struct cg_gemm* cg_gemm_t;
+int cg_gemm_create(const double* coef, int64_t n_coef, int32_t off_smh, const double* tmid, int32_t ns,
+                   int32_t nr, int32_t ntb, const double* nu_r, const double* nu_p, const double* eff,
+                   const int32_t* rec, const double* dnu, const int32_t* plog, int32_t n_plog,
+                   int64_t slab, cg_gemm_t* out);
+int cg_getrates_gemm(cg_gemm_t g, const double* P, const double* T, int64_t ng, int64_t ld,
+                     const double* Y, double* wdot, void* stream);
+int cg_gemm_free(cg_gemm_t g);
+
 #ifdef __cplusplus
 }
 #endif
