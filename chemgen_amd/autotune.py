@@ -52,7 +52,10 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
     straight-line lane-per-point and reaction-split kernels while the species state of a
     64-point tile fits (registers / LDS), table-driven kernels with every tile length whose
     LDS fits.  Every candidate divides with one Newton step after the reciprocal estimate
-    (``fdiv``: 1.9% faster on the GRI-sized split kernel, the same parity statistics)."""
+    (``fdiv``: 1.9% faster on the GRI-sized split kernel, the same parity statistics); table
+    kernels of 16- and 8-point tiles broadcast per-reaction data across each lane group
+    (``bcast``: 6% faster on the JetSurf-sized mechanism and on Hashemi 2016, 16% on Fuller 2021,
+    the same sums in the same order)."""
     return [dataclasses.replace(v, fdiv=True) for v in _candidates(mech)]
 
 
@@ -86,7 +89,8 @@ def _candidates(mech: Mechanism) -> List[Variant]:
         for npt in short + (16, 32, 64):
             for waves, wpe in ((4, 1), (8, 2), (16, 4)):
                 if table_fits(ns, nr, npt, waves):
-                    out.append(Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt))
+                    out.append(Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt,
+                                       bcast=npt in (8, 16)))
     return out
 
 
@@ -117,7 +121,8 @@ def default_variant(mech: Mechanism) -> Variant:
     straight-line kernels and runs 1.4-3.6x faster in the table kernel), with the longest tile of
     at most 16 points whose species state fits the LDS with a chunk of at least 512 reactions
     (or all of them), and the most wavefronts that fit with it (16, 4 per SIMD, won at every
-    tile length on the JetSurf-sized mechanism and Fuller 2021)."""
+    tile length on the JetSurf-sized mechanism and Fuller 2021); 16- and 8-point tiles
+    broadcast per-reaction data across each lane group (``bcast``)."""
     ns = mech.n_species
     if ns <= 24 or getattr(mech, "qss", ()):
         return Variant()
@@ -128,11 +133,13 @@ def default_variant(mech: Mechanism) -> Variant:
     for npt in (16, 8, 4, 2, 1):
         for waves, wpe in ((16, 4), (8, 2), (4, 1)):
             if table_fits(ns, mech.n_reactions, npt, waves, 512):
-                return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt, fdiv=True)
+                return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt, fdiv=True,
+                               bcast=npt in (8, 16))
     for npt in (16, 8, 4, 2, 1):
         for waves, wpe in ((16, 4), (8, 2), (4, 1)):
             if table_fits(ns, mech.n_reactions, npt, waves):
-                return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt, fdiv=True)
+                return Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt, fdiv=True,
+                               bcast=npt in (8, 16))
     raise ValueError(f"{ns} species: the table kernel's one-point tile does not fit the LDS")
 
 

@@ -89,12 +89,16 @@ class Variant:
                              # wavefront has two independent FMA chains in flight
     fdiv: bool = False       # divisions inside log2, h_k = C_k / EG_k and the falloff blending by
                              # a reciprocal with one Newton step (cg_div_fast, a few ulp)
+    bcast: bool = False      # table kernel of 16- or 8-point tiles: per-reaction data (records,
+                             # coefficients, efficiencies, PLOG pressures of the compiled class
+                             # loops; gather stream words) loaded once per lane group and
+                             # broadcast across its lanes (DPP), not loaded by every lane
 
     @property
     def tag(self) -> str:
         if self.table:
             return (f"t{self.table}s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.poly[0]}"
-                    + ("d" if self.fdiv else ""))
+                    + ("d" if self.fdiv else "") + ("b" if self.bcast else ""))
         c = (("k" if self.baked else "") + ("o" if self.live else "") + ("p" if self.pair else "")
              + ("d" if self.fdiv else ""))
         if self.split > 1:
@@ -1243,7 +1247,7 @@ def table_gather_streams(mech: Mechanism, chunks, goff: List[int], grow: List[in
                 best = (rows, lens)
         rows, lens = best
         nsl = len(lens)
-        steps = -(-sum(lens) // 4) * 4
+        steps = -(-sum(lens) // 16) * 16
         rec = dict(lens=lens, steps=steps, str_base=len(stream), spk_base=len(spk), comb_base=len(comb),
                    adds=[any(r[2][g][0] >= 0 and first[r[2][g][0]] != c for r in rows[W * s_:W * s_ + W]
                              for g in range(G)) for s_ in range(nsl)])
@@ -1276,11 +1280,17 @@ def table_gather_streams(mech: Mechanism, chunks, goff: List[int], grow: List[in
     return dict(chunks=out, stream=stream, spk=spk, comb=comb, fnu=fnu if frac else None, G=G)
 
 
-def _gather_code(ci: int, rec: Dict, frac: bool) -> str:
+def _gather_code(ci: int, rec: Dict, frac: bool, bcast: int = 0) -> str:
     """Straight-line gather of chunk ``ci`` for the table kernel (see ``table_gather_streams``):
     writer words, the first block of the lane group's entry stream, the writers' old wdot
     (addresses clamped into the arrays, so every load is unconditional), then the slot sums
-    with static register indices, each slot's store right after its last step."""
+    with static register indices, each slot's store right after its last step.  ``bcast``
+    (the lane group's size, 16 or 8): each lane loads one word of a block of ``bcast`` steps
+    and step t takes lane t mod ``bcast``'s word by a cross-lane broadcast (DPP), instead of
+    every lane of the group loading every word (``bcast`` times fewer bytes through the
+    vector memory pipeline, which the redundant loads saturate)."""
+    if bcast:
+        return _gather_code_bcast(ci, rec, frac, bcast)
     lens, steps = rec["lens"], rec["steps"]
     nsl = len(lens)
     L = [f"  {{  // gather of chunk {ci}: {nsl} slot(s) of {', '.join(map(str, lens))} steps",
@@ -1296,7 +1306,7 @@ def _gather_code(ci: int, rec: Dict, frac: bool) -> str:
     # that the first steps do not wait for those (scheduling barriers keep the order, and keep
     # the scheduler from hoisting every load to the top, which spills)
     order = _gather_order(lens)
-    nb = steps // 4
+    nb = -(-len(order) // 4)
     blk = 4
     def load_block(b0):
         return [f"    const uint4 v{b} = st[{b}];" for b in range(b0, min(nb, b0 + blk))]
@@ -1322,6 +1332,51 @@ def _gather_code(ci: int, rec: Dict, frac: bool) -> str:
             wd = f"v{t // 4}.{'xyzw'[t % 4]}"
             nu = f"sf[{t}]" if frac else f"CG_GNU({wd})"
             L.append(f"    a{s_} = __builtin_fma({nu}, CG_GQ({wd}), a{s_});")
+            if last[s_] == t:
+                L.append(f"    if (cm & {1 << s_}) {{")
+                L.append(f"      for (int o = CG_NPT; o < 64; o <<= 1) a{s_} += __shfl_xor(a{s_}, o);")
+                L.append("    }")
+                L.append(f"    if (k{s_} >= 0 && valid) *w{s_} = (k{s_} & (1 << 30)) ? a{s_} : p{s_} + a{s_};")
+    L.append("  }")
+    return "\n".join(L) + "\n"
+
+
+def _gather_code_bcast(ci: int, rec: Dict, frac: bool, B: int) -> str:
+    """``_gather_code`` with the stream words broadcast across the lane group (B lanes)."""
+    lens, steps = rec["lens"], rec["steps"]
+    nsl = len(lens)
+    L = [f"  {{  // gather of chunk {ci}: {nsl} slot(s) of {', '.join(map(str, lens))} steps",
+         f"    const int lg_ = cg_gw * CG_GG + cg_gg;",
+         f"    const int* sk = cg_gspk + {rec['spk_base']} + lg_ * {nsl};",
+         f"    const int cm = cg_gcomb[{rec['comb_base']} + cg_gw];"]
+    L += [f"    const int k{s_} = sk[{s_}];" for s_ in range(nsl)]
+    # lane pt of the group holds word B b + pt of block b (steps is a multiple of 16 >= B nb)
+    L.append(f"    const unsigned* st = cg_gstr + {rec['str_base']} + lg_ * {steps} + pt;")
+    if frac:
+        L.append(f"    const double* sf = cg_gfnu + {rec['str_base']} + lg_ * {steps};")
+    order = _gather_order(lens)
+    nb = -(-len(order) // B)
+    per = 16 // B                     # blocks per 16 steps, loaded one such group ahead
+    def load_block(b0):
+        return [f"    const unsigned u{b} = st[{B * b}];" for b in range(b0, min(nb, b0 + per))]
+    L += load_block(0)
+    L.append("    __builtin_amdgcn_sched_barrier(0);")
+    L += [f"    double* const w{s_} = cg_gwc + (uint64_t)(unsigned)(k{s_} >= 0 ? (k{s_} & 0xffffff) : 0) * cg_gld;"
+          for s_ in range(nsl)]
+    adds = rec["adds"]
+    L += [f"    const double p{s_} = *w{s_};" if adds[s_] else f"    const double p{s_} = 0.0;" for s_ in range(nsl)]
+    L += [f"    double a{s_} = 0.0;" for s_ in range(nsl)]
+    last = {}
+    for t, (s_, u) in enumerate(order):
+        last[s_] = t
+    for b0 in range(0, nb, per):
+        L.append("    __builtin_amdgcn_sched_barrier(0);")
+        L += load_block(b0 + per)
+        for t in range(B * b0, min(len(order), B * (b0 + per))):
+            s_ = order[t][0]
+            L.append(f"    const unsigned x{t} = CG_BCW(u{t // B}, {t % B});")
+            nu = f"sf[{t}]" if frac else f"CG_GNU(x{t})"
+            L.append(f"    a{s_} = __builtin_fma({nu}, CG_GQ(x{t}), a{s_});")
             if last[s_] == t:
                 L.append(f"    if (cm & {1 << s_}) {{")
                 L.append(f"      for (int o = CG_NPT; o < 64; o <<= 1) a{s_} += __shfl_xor(a{s_}, o);")
@@ -1372,17 +1427,29 @@ def _emit_table(mech: Mechanism, variant: Variant):
     plan = table_gather_plan(mech, npt, W)
     lay, rc, lds = plan["layout"], plan["rc"], plan["lds"]
     rows, effsp, segs, gen_start = lay.rows, lay.effsp, lay.segs, lay.gen_start
+
+    def most(a: int, b: int):
+        """(most collision efficiencies, most PLOG pressure points) of table positions [a, b)."""
+        rs = [mech.reactions[lay.order[t]] for t in range(a, b)]
+        ne = max([len(r.third_body) for r in rs if r.rtype in ("third_body", "lindemann", "troe", "sri")],
+                 default=0)
+        npt_ = max([len(set(c[0] for c in r.plog)) for r in rs if r.rtype == "plog"], default=1)
+        return ne, npt_
+
     classes = []
     for (a, b, (fam, rev, nrs, nps, dnu, _neg)) in segs:
         pf = None
         if rev:
             pf = {0: None, 1: "pinv", -1: "pfac"}.get(dnu, f"cg_exp2({dlit(-float(dnu))} * lg2pfac)")
+        nemax, npmax = most(a, b)
         classes.append(dict(start=a, end=b, fam=fam, rev=rev, nrs=nrs, nps=nps, dnu=dnu, pf=pf,
+                            nemax=nemax, npmax=npmax,
                             name=f"family {fam}, {'reversible' if rev else 'irreversible'}, "
                                  f"{nrs} -> {nps} slots, dnu {dnu}"))
     chunk_list, gs = plan["chunks"], plan["streams"]
     gfrac = bool(gs and gs["fnu"])
-    chunks = [dict(c0=c0, c1=c1, gather=_gather_code(ci, gs["chunks"][ci], gfrac),
+    bc = npt if (variant.bcast and npt in (8, 16)) else 0
+    chunks = [dict(c0=c0, c1=c1, gather=_gather_code(ci, gs["chunks"][ci], gfrac, bc),
                    pieces=[dict(a=a, b=b, cl=classes[si] if si < len(classes) else None)
                                          for (a, b, si) in pieces])
               for ci, (c0, c1, pieces) in enumerate(chunk_list)]
@@ -1396,7 +1463,7 @@ def _emit_table(mech: Mechanism, variant: Variant):
         hash_hex=f"0x{h:016x}", off_invmw=0, off_smh=off_smh, R0=dlit(R0),
         lg2_patm_r0=dlit(math.log2(PATM / R0)), patm_r0=dlit(PATM / R0), r0_patm=dlit(R0 / PATM),
         log10_2=dlit(LOG10_2), log2_10=dlit(LOG2_10), exp_impl=variant.exp_impl, poly=variant.poly,
-        fdiv=variant.fdiv, rtab=rtab, effsp=", ".join(str(k) for k in effsp) or "0", n_effsp=max(1, len(effsp)),
+        fdiv=variant.fdiv, bcast=bc, rtab=rtab, effsp=", ".join(str(k) for k in effsp + [mech.n_species]), n_effsp=len(effsp) + 1,
         classes=classes, gen_start=gen_start, chunks=chunks, rc=rc,
         gG=gs["G"] if gs else 1,
         gspk=", ".join(str(v) for v in gs["spk"]) if gs and gs["spk"] else "-1",

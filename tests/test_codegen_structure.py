@@ -58,3 +58,40 @@ def test_fdiv_variant_divides_with_one_newton_step():
         assert "cg_div_fast(m - 1.0, m + 1.0)" in fast and "cg_div(m - 1.0, m + 1.0)" in slow
         assert fast.count("cg_div_fast(") > slow.count("cg_div_fast(") + 3, v.tag
         assert dataclasses.replace(v, fdiv=True).tag == v.tag + "d"
+
+
+def test_bcast_table_kernel_reads_each_word_from_its_lane():
+    """The broadcast table kernel (16- and 8-point tiles): in the gather, lane pt of a group
+    loads word B b + pt of block b and step t takes lane t mod B of block t div B, so step t
+    reads stream word t; in the class rounds, lane pt loads the record of round n0 + pt and
+    round n takes lane n - n0.  Checked on the generated source of both tile lengths."""
+    import re
+    from chemgen_amd.autotune import candidate_variants
+    m = load_bundled("synth_jetsurf348")
+    for npt in (16, 8):
+        v = Variant(64, 4, "lean", "gh", "horner", False, 4, 16, npt, fdiv=True, bcast=True)
+        assert v in candidate_variants(m) or npt == 8
+        src = emit_hip(m, v)[0]
+        assert "cg_bci<" in src and "0x158 + N" in src if npt == 8 else "0x150 + N" in src
+        gathers = src.split("// gather of chunk ")[1:]
+        assert gathers
+        for g in gathers:
+            g = g.split("\n  }\n")[0]
+            for b, off in re.findall(r"const unsigned u(\d+) = st\[(\d+)\];", g):
+                assert int(off) == npt * int(b)
+            steps = [(int(t), int(u), int(n)) for t, u, n in
+                     re.findall(r"const unsigned x(\d+) = CG_BCW\(u(\d+), (\d+)\);", g)]
+            assert steps and [t for t, _, _ in steps] == list(range(len(steps)))
+            assert all(u == t // npt and n == t % npt for t, u, n in steps)
+        rounds = src.split("round(s)\n")[1:]
+        assert rounds
+        for r in rounds:
+            blocks = re.findall(r"const int jl = min\(j0 \+ \((\d+) \+ pt\) \* CG_SLOTS", r.split("  }\n  {")[0])
+            for n0 in blocks:
+                body = r.split(f"const int jl = min(j0 + ({n0} + pt) * CG_SLOTS")[1]
+                body = body.split("const int jl = min(")[0]
+                for n, m_ in re.findall(r"const int j = j0 \+ (\d+) \* CG_SLOTS;\n\s+const uint32_t ry = CG_BCI\(lt\.y, (\d+)\)", body):
+                    assert int(n) - int(n0) == int(m_) and int(m_) < npt
+    # 32-point tiles (a group wider than a DPP row) keep the per-lane loads
+    v32 = [v for v in candidate_variants(m) if v.table == 32]
+    assert all(not v.bcast for v in v32)

@@ -86,9 +86,9 @@ def test_ragged_batch_matches_oracle(name, grid):
 
 @pytest.mark.parametrize("name,tags", [
     ("synth_gri53", ("b64w1lghhd", "b64w1lghhkod", "s4w2lghhg8d", "s4w3lghhg2d", "s8w4lghhg4d")),
-    ("gri30", ("t16s8w2lhd", "b64w2lghhL4d")),
+    ("gri30", ("t16s8w2lhdb", "b64w2lghhL4d")),
     ("synth_ext53", ("t16s4w1lh", "t4s8w2lh")),
-    ("fuller2021", ("t8s16w4lhd", "t2s8w2lhd", "t1s4w1lhd")),
+    ("fuller2021", ("t8s16w4lhdb", "t2s8w2lhd", "t1s4w1lhd")),
 ])
 def test_ragged_batch_kernel_families(name, tags):
     """Both batched kernels (persistent walk, one workgroup per tile) of each kernel family
