@@ -55,7 +55,9 @@ def candidate_variants(mech: Mechanism) -> List[Variant]:
     (``fdiv``: 1.9% faster on the GRI-sized split kernel, the same parity statistics); table
     kernels of 16- and 8-point tiles broadcast per-reaction data across each lane group
     (``bcast``: 6% faster on the JetSurf-sized mechanism and on Hashemi 2016, 16% on Fuller 2021,
-    the same sums in the same order)."""
+    the same sums in the same order).  Table workgroups of 12 wavefronts (3 per SIMD, 168
+    registers) sit between 8 and 16: the JetSurf-sized kernel spills at 16 (128 registers) and
+    runs 14% faster at 12."""
     return [dataclasses.replace(v, fdiv=True) for v in _candidates(mech)]
 
 
@@ -87,7 +89,7 @@ def _candidates(mech: Mechanism) -> List[Variant]:
         nr = mech.n_reactions
         short = (1, 2, 4, 8) if not table_fits(ns, nr, 16, 4, 512) else ()
         for npt in short + (16, 32, 64):
-            for waves, wpe in ((4, 1), (8, 2), (16, 4)):
+            for waves, wpe in ((4, 1), (8, 2), (12, 3), (16, 4)):
                 if table_fits(ns, nr, npt, waves):
                     out.append(Variant(64, wpe, "lean", "gh", "horner", False, 4, waves, npt,
                                        bcast=npt in (8, 16)))
