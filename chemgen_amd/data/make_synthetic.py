@@ -20,6 +20,8 @@ Outputs (Chemkin text, thermo inline):
   synth_plog53.inp      GRI30 species; 120 reactions: 30 PLOG, 8 Troe, 12 third-body
   synth_ext53.inp       synth_plog53 plus 12 reactions with sides of 4+ species slots or
                         coefficients 0.5 (sums and halves of its own reactions, every kind)
+  synth_eff53.inp       synth_gri53 with 9-15 collision efficiencies on every third-body and
+                        falloff reaction (from synth_gri53.inp alone: no ck_files needed)
 
     python -m chemgen_amd.data.make_synthetic /path/to/reference/ck_files synth_plog53
 
@@ -61,6 +63,13 @@ def main(ck_files: str, only=()) -> None:
         m = synthetic_mechanism(base.species, base.elements, name=name, **SPECS[name])
         m2 = write_and_parse(m, os.path.join(HERE, name + ".inp"))
         print(name, m2.n_species, m2.n_reactions, m2.counts())
+    if not only or "synth_eff53" in only:
+        from ..synth import with_long_efficiency_lists
+        from ..mechanism import parse_chemkin
+        base = parse_chemkin(os.path.join(HERE, "synth_gri53.inp"), name="synth_gri53")
+        m = with_long_efficiency_lists(base, 9, 15, seed=5354, name="synth_eff53")
+        m2 = write_and_parse(m, os.path.join(HERE, "synth_eff53.inp"))
+        print("synth_eff53", m2.n_species, m2.n_reactions, m2.counts())
     if not only or "synth_ext53" in only:
         from ..synth import with_extended_reactions
         from ..mechanism import parse_chemkin

@@ -29,6 +29,9 @@ TABLE_CHECKED = {
                     Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
     "hashemi2016": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 16),
                     Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 4)),
+    # compiled third-body and Troe rounds with 9-15 efficiencies: two loads per 8-lane group
+    "synth_eff53": (Variant(64, 1, "lean", "gh", "horner", False, 4, 4, 8, fdiv=True, bcast=True),
+                    Variant(64, 2, "lean", "gh", "horner", False, 4, 8, 16, fdiv=True, bcast=True)),
 }
 
 

@@ -213,6 +213,25 @@ def with_extended_reactions(mech: Mechanism, n: int, seed: int, name: str) -> Me
     return Mechanism(name, list(mech.elements), list(mech.species), base + extra)
 
 
+def with_long_efficiency_lists(mech: Mechanism, lo: int, hi: int, seed: int, name: str) -> Mechanism:
+    """``mech`` with every third-body and falloff reaction given between ``lo`` and ``hi``
+    collision efficiencies (eff - 1 in [-0.7, 5], as the parser stores them) on species drawn
+    at random: efficiency lists longer than a lane group of 8 lanes, which the broadcast table
+    kernel of 8-point tiles loads in two parts."""
+    import copy
+    rng = np.random.default_rng(seed)
+    names = [sp.name for sp in mech.species]
+    out: List[Reaction] = []
+    for r in mech.reactions:
+        r = copy.deepcopy(r)
+        if r.rtype in ("third_body", "troe", "lindemann", "sri"):
+            k = int(rng.integers(lo, hi + 1))
+            pick = rng.choice(len(names), size=k, replace=False)
+            r.third_body = {names[i]: float(np.round(rng.uniform(-0.7, 5.0), 3)) for i in sorted(pick)}
+        out.append(r)
+    return Mechanism(name, list(mech.elements), list(mech.species), out)
+
+
 def write_and_parse(mech: Mechanism, path: str) -> Mechanism:
     write_chemkin(mech, path)
     return parse_chemkin(path, name=mech.name)

@@ -65,15 +65,17 @@ def test_ragged_tail_sizes(torch_cuda, ng):
 
 
 @pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53", "synth_ext53",
-                                  "fuller2021", "hashemi2016", "ic8_llnl"])
+                                  "fuller2021", "hashemi2016", "ic8_llnl", "synth_eff53"])
 def test_table_kernels_vs_oracle(torch_cuda, name):
     """Every table-driven kernel built for the mechanism (all tile lengths and wavefront counts
     that fit the LDS) against the oracle; gri30 brings Troe with and without T2 and Lindemann
     reactions, the JetSurf-sized mechanism SRI, synth_plog53 PLOG reactions (in compiled class
     loops and the generic loop), synth_ext53 extended records (sides of 4-5 species slots,
     coefficients 0.5, with PLOG, third-body and falloff rates), Fuller 2021 (835 species) all of
-    them in 1- to 8-point tiles, Hashemi 2016 duplicate reactions with negative prefactors;
-    300 points leave a partial tile for every tile length."""
+    them in 1- to 8-point tiles, Hashemi 2016 duplicate reactions with negative prefactors,
+    synth_eff53 compiled third-body and Troe rounds whose 9-15 efficiencies the broadcast
+    kernel of 8-point tiles loads in two parts; 300 points leave a partial tile for every tile
+    length."""
     from chemgen_amd.kernels import default_variants
     from chemgen_amd.runtime import DeviceMechanism
     m = load_bundled(name)
