@@ -47,6 +47,19 @@ def table_fits(ns: int, nr: int, npt: int, waves: int, min_chunk: int = 0) -> bo
             and table_chunk_reactions(ns, nr, npt, waves) >= min(nr, min_chunk))
 
 
+def two_tile_points(ns: int) -> int:
+    """Points per reaction-split tile (a multiple of 8, 48 or more) that lets two tiles' species
+    state (2 x ns x points doubles each, in 512-byte LDS granules) share a CU where 64-point
+    tiles cannot; 0 when 64 points already fit twice or 48 do not."""
+    gran = lambda b: -(-b // 512) * 512
+    if 2 * gran(8 * 2 * ns * 64) <= LDS_BYTES:
+        return 0
+    for tp in range(56, 47, -8):
+        if 2 * gran(8 * 2 * ns * tp) <= LDS_BYTES:
+            return tp
+    return 0
+
+
 def candidate_variants(mech: Mechanism) -> List[Variant]:
     """Candidates swept by default (all compiled by the build for the bundled mechanisms):
     straight-line lane-per-point and reaction-split kernels while the species state of a
@@ -76,6 +89,13 @@ def _candidates(mech: Mechanism) -> List[Variant]:
                     out.append(Variant(64, waves, "lean", "gh", "horner", False, group, split))
                 # the same with the coefficients compiled in
                 out.append(Variant(64, waves, "lean", "gh", "horner", False, groups[-1], split, 0, True))
+        # 8-wavefront tiles shortened so that two tiles share a CU where one 64-point tile alone
+        # fills more than half the LDS (n-heptane-sized, 88 species: 56 points, 4 waves/SIMD)
+        tp2 = two_tile_points(ns)
+        if tp2 and mech.n_reactions >= 8:
+            for group in (2, 4):
+                for baked in (False, True):
+                    out.append(Variant(64, 4, "lean", "gh", "horner", False, group, 8, 0, baked, tp=tp2))
         out.append(Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, True))
         # lane per point with reactions in live-range order, coefficients read or compiled in
         out += [Variant(64, 1, "lean", "gh", "horner", False, 4, 1, 0, False, True),

@@ -89,6 +89,9 @@ class Variant:
                              # wavefront has two independent FMA chains in flight
     fdiv: bool = False       # divisions inside log2, h_k = C_k / EG_k and the falloff blending by
                              # a reciprocal with one Newton step (cg_div_fast, a few ulp)
+    tp: int = 64             # reaction-split tiles of tp < 64 points: lanes tp..63 repeat lanes
+                             # 0..63-tp (same values, same LDS writes), so that the species state
+                             # of one more tile fits the LDS of a CU
     bcast: bool = False      # table kernel of 16- or 8-point tiles: per-reaction data (records,
                              # coefficients, efficiencies, PLOG pressures of the compiled class
                              # loops; gather stream words) loaded once per lane group and
@@ -103,7 +106,7 @@ class Variant:
              + ("d" if self.fdiv else ""))
         if self.split > 1:
             return (f"s{self.split}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
-                    f"{self.poly[0]}g{self.group}{c}")
+                    f"{self.poly[0]}g{self.group}{c}" + (f"n{self.tp}" if self.tp != 64 else ""))
         return (f"b{self.block}w{self.waves_per_eu}{self.exp_impl[0]}{self.kr_mode}"
                 f"{self.poly[0]}{'L%d' % self.group if self.g_lds else ''}{c}")
 
@@ -116,7 +119,7 @@ class Variant:
         """Grid points per workgroup."""
         if self.table:
             return self.table
-        return 64 if self.split > 1 else self.block
+        return self.tp if self.split > 1 else self.block
 
 
 class _Coef:
@@ -879,9 +882,12 @@ def _emit_split(mech: Mechanism, variant: Variant):
     parts = partition_reactions(mech, W)
     # the per-wave partial sums get LDS rows of their own unless that LDS would keep the
     # workgroups per CU below the occupancy the registers are sized for
-    lds_full = 8 * (2 * ns * 64 + W * 64)
+    tp = variant.tp
+    if not 32 <= tp <= 64:
+        raise ValueError("split kernel: 32 to 64 points per tile")
+    lds_full = 8 * (2 * ns * tp + W * tp)
     compact_lds = (W <= ns and (LDS_PER_CU // lds_full) * W < 4 * variant.waves_per_eu
-                   and (LDS_PER_CU // (8 * 2 * ns * 64)) > LDS_PER_CU // lds_full)
+                   and (LDS_PER_CU // (8 * 2 * ns * tp)) > LDS_PER_CU // lds_full)
     codes, touched = [], []
     for v in range(W):
         lines: List[str] = []
@@ -915,7 +921,7 @@ def _emit_split(mech: Mechanism, variant: Variant):
         r0_patm=dlit(R0 / PATM), exp_impl=variant.exp_impl, kr_mode=variant.kr_mode, poly=variant.poly,
         tmid_groups=[{"tmid": dlit(t)} for t in tmids], tmid_group_of=group_of, species_of=species_of,
         zero_of=species_of, touched=touched, reactions_code=codes, compact_lds=compact_lds,
-        pair=variant.pair, fdiv=variant.fdiv,
+        pair=variant.pair, fdiv=variant.fdiv, tp=tp,
         **_baked_args(coef, variant),
     )
     meta = dict(ns=ns, nr=mech.n_reactions, ncoef=len(coef), block=variant.threads, ppb=variant.ppb,

@@ -291,9 +291,10 @@ static int tiles_launch(cg_mech* h, const int64_t* tile_start, const int32_t* ti
                 (long long)n_points);
   if (ld >= (1LL << 32)) return fail(CG_ERANGE, "%s: ld must be below 2^32", who);
   // grid > 0: that many persistent workgroups; < 0: one workgroup per tile; 0: one per tile
-  // for tiles shorter than 64 points (table kernels: one tile is ~1e5 cycles of work, and a
-  // persistent walk over ragged tiles measured 1.2-1.3x the time), else 8 per CU
-  int64_t g = grid > 0 ? grid : ((grid < 0 || h->meta.ppb < 64) ? nt : (int64_t)h->n_cu * 8);
+  // for tiles of at most 16 points (the table kernels of large mechanisms: one tile is ~1e5
+  // cycles of work, and a persistent walk over ragged tiles measured 1.2-1.3x the time), else
+  // 8 per CU
+  int64_t g = grid > 0 ? grid : ((grid < 0 || h->meta.ppb <= 16) ? nt : (int64_t)h->n_cu * 8);
   if (g > nt) g = nt;
   if (g > 0x7fffffffLL) g = 0x7fffffffLL;
   void* args[] = {(void*)&P, (void*)&T, (void*)&Y, (void*)&W, (void*)&h->coef_d,

@@ -75,8 +75,8 @@ int cg_getrates(cg_mech_t h, const double* P, const double* T, int64_t ng, int64
  * into tiles of at most the kernel's points per workgroup (never straddling a patch, aligned
  * to tile boundaries of the packed arrays) and one launch walks the tile list.  `grid` =
  * number of persistent workgroups walking the list (0 = the default: 8 per CU, or one
- * workgroup per tile for kernels whose tiles are shorter than 64 points -- the table kernels
- * of large mechanisms, where one tile is tens of microseconds of work and the hardware's
+ * workgroup per tile for kernels whose tiles hold at most 16 points -- the table kernels of
+ * large mechanisms, where one tile is tens of microseconds of work and the hardware's
  * dispatch balances ragged tiles better than a persistent walk); negative, or at least the
  * number of tiles, = one workgroup per tile (a loop-free kernel over the same list). */
 

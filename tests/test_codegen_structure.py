@@ -95,3 +95,22 @@ def test_bcast_table_kernel_reads_each_word_from_its_lane():
     # 32-point tiles (a group wider than a DPP row) keep the per-lane loads
     v32 = [v for v in candidate_variants(m) if v.table == 32]
     assert all(not v.bcast for v in v32)
+
+
+def test_short_split_tiles_repeat_lanes_and_fit_two_per_cu():
+    """Reaction-split tiles of tp < 64 points: lanes tp..63 work on columns 0..63-tp (same
+    values, same LDS writes), the LDS rows are tp wide, the tile length in the metadata is tp,
+    and two tiles of the n-heptane-sized mechanism's state fit the 160 KB of a CU at 56 points
+    (not at 64)."""
+    from chemgen_amd.autotune import LDS_BYTES, candidate_variants, two_tile_points
+    m = load_bundled("synth_nhept88")
+    assert two_tile_points(m.n_species) == 56
+    assert two_tile_points(53) == 0                     # 64-point tiles already fit twice
+    gran = lambda b: -(-b // 512) * 512
+    assert 2 * gran(8 * 2 * 88 * 64) > LDS_BYTES >= 2 * gran(8 * 2 * 88 * 56)
+    v = next(v for v in candidate_variants(m) if v.tp == 56 and v.baked and v.group == 2)
+    assert v.tag == "s8w4lghhg2kdn56" and v.ppb == 56
+    src, _, meta = emit_hip(m, v)
+    assert "#define CG_TP 56" in src and "lane < CG_TP ? lane : lane - CG_TP" in src
+    assert "* 64]" not in src and meta["ppb"] == 56 and meta["compact_lds"]
+    assert "#define CG_SMEM_DOUBLES (2 * CG_NS * CG_TP)" in src

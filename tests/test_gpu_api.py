@@ -89,13 +89,14 @@ def test_ragged_batch_matches_oracle(name, grid):
     ("gri30", ("t16s8w2lhdb", "b64w2lghhL4d")),
     ("synth_ext53", ("t16s4w1lh", "t4s8w2lh")),
     ("fuller2021", ("t8s16w4lhdb", "t2s8w2lhd", "t1s4w1lhd")),
+    ("synth_nhept88", ("s8w4lghhg2kdn56",)),
 ])
 def test_ragged_batch_kernel_families(name, tags):
     """Both batched kernels (persistent walk, one workgroup per tile) of each kernel family
     -- lane, live-range lane with compiled-in coefficients, reaction-split (also with the
-    compact LDS layout), table (also with PLOG and extended records, and with the 1- to
-    8-point tiles of an 835-species mechanism), EG in LDS -- against the oracle on ragged
-    patches."""
+    compact LDS layout, and with 56-point tiles whose last 8 lanes repeat the first 8), table
+    (also with PLOG and extended records, and with the 1- to 8-point tiles of an 835-species
+    mechanism), EG in LDS -- against the oracle on ragged patches."""
     import torch
     from chemgen_amd.autotune import candidate_variants
     from chemgen_amd.kernels import default_variants
