@@ -207,7 +207,7 @@ def bench_batched(steps, warmup):
     sizes = np.diff(offs)
     return {"workload": f"{name} ({mech.n_species} species, {mech.n_reactions} reactions): ragged batch of "
                         f"{len(sizes)} patches, log-uniform 2^8-2^17 points, {ng} points in total, "
-                        f"T U(600,2800) K, seed 0; one persistent launch replayed from a HIP graph",
+                        f"T U(600,2800) K, seed 0; one launch over the tile list replayed from a HIP graph",
             "points_per_s": ng / t_graph, "kernel_avg_ms": t_graph * 1e3,
             "contiguous_points_per_s": ng / t_plain, "kernel_variant": variant.tag,
             "grid": grid if grid > 0 else ("8 x CUs" if grid == 0 else "one workgroup per tile"), "patch_min": int(sizes.min()), "patch_max": int(sizes.max()),
