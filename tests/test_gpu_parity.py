@@ -64,6 +64,21 @@ def test_ragged_tail_sizes(torch_cuda, ng):
     assert parity_stats(m, w, ref, qf, qb)["max_gross"] <= TOL_GROSS
 
 
+@pytest.mark.parametrize("ng", [1, 55, 56, 57, 113])
+def test_short_split_tile_tails(torch_cuda, ng):
+    """The n-heptane-sized mechanism's tuned kernel has 56-point tiles (lanes 56..63 repeat
+    lanes 0..7): tails of every length around the tile, against the oracle."""
+    from chemgen_amd.autotune import best_variant
+    m = load_bundled("synth_nhept88")
+    assert best_variant(m).ppb == 56
+    print(f"[tp56] ng={ng}", flush=True)
+    P, T, Y = make_states(m.n_species, ng, seed=ng, T_range=(600.0, 2800.0))
+    w = _run(torch_cuda, m, P, T, Y)
+    ref, qf, qb = oracle_getrates(m, P, T, Y, return_parts=True)
+    st = parity_stats(m, w, ref, qf, qb)
+    assert st["finite"] and st["max_gross"] <= TOL_GROSS, st
+
+
 @pytest.mark.parametrize("name", ["gri30", "synth_gri53", "synth_jetsurf348", "synth_plog53", "synth_ext53",
                                   "fuller2021", "hashemi2016", "ic8_llnl", "synth_eff53"])
 def test_table_kernels_vs_oracle(torch_cuda, name):
