@@ -458,14 +458,17 @@ __device__ __forceinline__ float infer_sinh(float z) {
   return fabsf(z) < 0.125f ? small : big;
 }
 
-template <int H>
+// PTV: 16-point tiles per wavefront, NWV: wavefronts per workgroup.  Two tiles and eight
+// wavefronts (256 points per workgroup, two wavefronts per SIMD) for H = 256 by default; fewer
+// cut a workgroup's latency, which pays when the batch does not fill the GPU.
+template <int H, int PTV = (H == 256 ? 2 : 1), int NWV = 8>
 struct InferShape {
-  static constexpr int NW = 8;                    // wavefronts per workgroup (two per SIMD)
+  static constexpr int NW = NWV;                  // wavefronts per workgroup
   static constexpr int KCH = H / 32;              // k-chunks of a hidden layer
   static constexpr int NTH = H / 16;              // output tiles of a hidden layer
   static constexpr int NG = 64 / KCH;             // output tiles per 64 KB weight chunk
   static constexpr int NCH = NTH / NG;            // chunks of a hidden layer
-  static constexpr int PT = H == 256 ? 2 : 1;     // 16-point tiles per wavefront
+  static constexpr int PT = PTV;                  // 16-point tiles per wavefront
   static constexpr int NP = 16 * PT;              // points per wavefront
   static constexpr int LPP = 64 / NP;             // lanes per point (features, decode)
   static constexpr int KP = 64 / (NW * PT);       // feature k-chunks staged per pass (64 KB)
@@ -482,10 +485,10 @@ typedef __attribute__((address_space(3))) void* cg_lptr;
 // ci & 1 (fragment f of the chunk -> LDS fragment slot f; wavefront w copies f = w, w+NW, ...).
 // Chunks: layer 0 (NCH of NG x kc0 fragments), hidden layers (NCH of NG x KCH), output
 // (ceil(nt_out / NG) of up to NG x KCH).
-template <int H>
+template <int H, int NWV>
 __device__ __forceinline__ void infer_prefetch(const bf16x8* __restrict__ Wf, int ci, int kc0, int n_hidden,
                                                int nt_out, bf16x8* wlds) {
-  using S = InferShape<H>;
+  using S = InferShape<H, 1, NWV>;
   int64_t off;
   int nfrag;
   if (ci < S::NCH) {
@@ -514,13 +517,13 @@ __device__ __forceinline__ void infer_prefetch(const bf16x8* __restrict__ Wf, in
 // (running index over the network) is in buffer ci & 1 and the next one is prefetched.  MODE
 // fixes which extent is known at compile time, so the MFMA blocks carry no per-fragment
 // branches: 0 = first layer (runtime kc0), 1 = hidden layer, 2 = output layer (runtime nt_out).
-template <int H, int MODE>
+template <int H, int PTV, int NWV, int MODE>
 __device__ __forceinline__ void infer_layer(const bf16x8* __restrict__ Wf, int& ci, int n_chunks, int kc0,
                                             int n_hidden, int nt_out, bf16x8* __restrict__ wlds,
                                             const float* __restrict__ bl,
-                                            const bf16x8 (&act)[H / 32][InferShape<H>::PT],
-                                            f32x4 (&acc)[H / 16][InferShape<H>::PT]) {
-  using S = InferShape<H>;
+                                            const bf16x8 (&act)[H / 32][PTV],
+                                            f32x4 (&acc)[H / 16][PTV]) {
+  using S = InferShape<H, PTV, NWV>;
   const int lane = threadIdx.x & 63;
   const int kc_n = MODE == 0 ? kc0 : S::KCH;
   const int nt_n = MODE == 2 ? nt_out : S::NTH;
@@ -537,7 +540,7 @@ __device__ __forceinline__ void infer_layer(const bf16x8* __restrict__ Wf, int& 
     if (c0 < nt_n) {
       __builtin_amdgcn_s_waitcnt(0);   // this wavefront's copies of chunk ci have landed
       __syncthreads();                 // ... everyone's, and chunk ci-1's buffer is free
-      if (ci + 1 < n_chunks) infer_prefetch<H>(Wf, ci + 1, kc0, n_hidden, nt_out, wlds);
+      if (ci + 1 < n_chunks) infer_prefetch<H, NWV>(Wf, ci + 1, kc0, n_hidden, nt_out, wlds);
       const bf16x8* __restrict__ buf = wlds + (ci & 1) * 4096;
       // Steps of Q tiles x one k-chunk; the fragments of step st+1 are loaded (into the other
       // half of F) before the MFMAs of step st are issued -- sched_barrier keeps that order --
@@ -575,10 +578,10 @@ __device__ __forceinline__ void infer_layer(const bf16x8* __restrict__ Wf, int& 
 
 // SiLU + BF16 rounding (the bias is already in acc): a hidden layer's results become the next
 // layer's operand
-template <int H>
-__device__ __forceinline__ void infer_activate(const f32x4 (&acc)[H / 16][InferShape<H>::PT],
-                                               bf16x8 (&act)[H / 32][InferShape<H>::PT]) {
-  using S = InferShape<H>;
+template <int H, int PTV>
+__device__ __forceinline__ void infer_activate(const f32x4 (&acc)[H / 16][PTV],
+                                               bf16x8 (&act)[H / 32][PTV]) {
+  using S = InferShape<H, PTV>;
 #pragma unroll
   for (int kc = 0; kc < S::KCH; ++kc) {
 #pragma unroll
@@ -596,14 +599,14 @@ __device__ __forceinline__ void infer_activate(const f32x4 (&acc)[H / 16][InferS
 
 // Wf: packed layers back to back (layer 0: NTH x kc0 fragments, hidden layers: NTH x KCH,
 // output: nt_out x KCH); bias: hidden layers H each, then the output layer's ns values.
-template <int H>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+template <int H, int PTV, int NWV>
+__global__ void __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(2, 2)))
 mlp_infer_k(const double* __restrict__ P, const double* __restrict__ T, const double* __restrict__ Y,
             int64_t ld, int64_t ng, int ns, const float* __restrict__ fs, const bf16x8* __restrict__ Wf,
             const float* __restrict__ bias, int kc0, int n_hidden, int nt_out, const double* __restrict__ s,
             const float* __restrict__ aek, int ne, double tol, double* __restrict__ W, int64_t ldw,
             int* __restrict__ flag, double* __restrict__ resid, const int* __restrict__ skip) {
-  using S = InferShape<H>;
+  using S = InferShape<H, PTV, NWV>;
   constexpr int PT = S::PT, NP = S::NP, LPP = S::LPP;
   // a device-side verdict (the surrogate's audit) can cancel the launch: every workgroup reads
   // it first, before any memory traffic, and leaves
@@ -616,7 +619,7 @@ mlp_infer_k(const double* __restrict__ P, const double* __restrict__ T, const do
   const int64_t mc = m < ng ? m : ng - 1;
   const int n_chunks = S::NCH * n_hidden + (nt_out + S::NG - 1) / S::NG;
   int ci = 0;
-  infer_prefetch<H>(Wf, 0, kc0, n_hidden, nt_out, wlds);   // lands in buffer 0 meanwhile
+  infer_prefetch<H, NWV>(Wf, 0, kc0, n_hidden, nt_out, wlds);   // lands in buffer 0 meanwhile
   // the decode's per-output constants (s_n, the output bias, the element matrix) in LDS past
   // the weight and decode buffers: read there by every output of every point, they had been
   // ten vector-memory loads per output (10-13% of the kernel's time)
@@ -674,13 +677,13 @@ mlp_infer_k(const double* __restrict__ P, const double* __restrict__ T, const do
 
   // 2. layers
   f32x4 acc[S::NTH][PT];
-  infer_layer<H, 0>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, bias, act, acc);
-  infer_activate<H>(acc, act);
+  infer_layer<H, PT, NWV, 0>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, bias, act, acc);
+  infer_activate<H, PT>(acc, act);
   for (int l = 1; l < n_hidden; ++l) {
-    infer_layer<H, 1>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, bias + l * H, act, acc);
-    infer_activate<H>(acc, act);
+    infer_layer<H, PT, NWV, 1>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, bias + l * H, act, acc);
+    infer_activate<H, PT>(acc, act);
   }
-  infer_layer<H, 2>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, nullptr, act, acc);
+  infer_layer<H, PT, NWV, 2>(Wf, ci, n_chunks, kc0, n_hidden, nt_out, wlds, nullptr, act, acc);
 
   // 3. decode through LDS, TD output tiles at a time: point p's z values are read by LPP lanes
   float* zl = (float*)wlds + wave * (NP * S::ZW);
@@ -941,12 +944,12 @@ extern "C" int cg_mlp_pack(const void* W, int32_t N, int32_t K, int32_t NT, int3
 }
 
 namespace {
-template <int H>
+template <int H, int PTV, int NWV>
 int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, int64_t ng, int32_t ns,
                  const float* fscale, const void* Wf, const float* bias, int32_t kc0, int32_t n_hidden,
                  int32_t nt_out, const double* s, const float* aek, int32_t ne, double tol, double* W,
                  int64_t ldw, int32_t* flag, double* resid, const int32_t* skip, hipStream_t stream) {
-  using S = InferShape<H>;
+  using S = InferShape<H, PTV, NWV>;
   // LDS: weight / decode buffers, then the decode constants (s_n as doubles, the output bias
   // and ne element rows as floats)
   constexpr int kMaxLds = 160 * 1024;
@@ -954,14 +957,14 @@ int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, 
   if (lds > kMaxLds) return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: too many species for the LDS");
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_k<H>,
+    hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_k<H, PTV, NWV>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
     if (e != hipSuccess) return cg_internal_fail(CG_EHIP, "cg_mlp_infer: cannot reserve LDS");
     attr_set = true;
   }
   const int64_t nb = (ng + S::NW * S::NP - 1) / (S::NW * S::NP);
   if (nb > 0x7fffffffLL) return cg_internal_fail(CG_ERANGE, "cg_mlp_infer: ng too large");
-  mlp_infer_k<H><<<(unsigned)nb, 64 * S::NW, (unsigned)lds, stream>>>(
+  mlp_infer_k<H, PTV, NWV><<<(unsigned)nb, 64 * S::NW, (unsigned)lds, stream>>>(
       P, T, Y, ld, ng, ns, fscale, (const bf16x8*)Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W, ldw,
       flag, resid, skip);
   CG_LAUNCH_CHECK("cg_mlp_infer launch");
@@ -982,9 +985,26 @@ extern "C" int cg_mlp_infer(const double* P, const double* T, const double* Y, i
   const int32_t kc0 = (kin + 31) / 32, nt_out = (ns + 15) / 16;
   if (kin < ns + 2 || kc0 > hidden / 32 || nt_out > hidden / 16)
     return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: input/output wider than the hidden layers");
-  if (hidden == 256)
-    return launch_infer<256>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
-                             ldw, flag, resid, skip, (hipStream_t)stream);
-  return launch_infer<512>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
+  if (hidden == 256) {
+    // a batch of at most one 128-point workgroup per CU (the surrogate's audit, small calls)
+    // finishes in one workgroup's latency: half-size tiles halve it
+    static int n_cu = 0;
+    if (n_cu == 0) {
+      int dev = 0, v = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+        return cg_internal_fail(CG_EHIP, "cg_mlp_infer: cannot read the CU count");
+      n_cu = v;
+    }
+    if (ng <= (int64_t)n_cu * 64)
+      return launch_infer<256, 1, 4>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol,
+                                     W, ldw, flag, resid, skip, (hipStream_t)stream);
+    if (ng <= (int64_t)n_cu * 128)
+      return launch_infer<256, 1, 8>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol,
+                                  W, ldw, flag, resid, skip, (hipStream_t)stream);
+    return launch_infer<256, 2, 8>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
+                                ldw, flag, resid, skip, (hipStream_t)stream);
+  }
+  return launch_infer<512, 1, 8>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol, W,
                            ldw, flag, resid, skip, (hipStream_t)stream);
 }

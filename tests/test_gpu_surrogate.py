@@ -271,7 +271,10 @@ def test_features_match_cpu_oracle(torch):
 
 
 @pytest.mark.parametrize("name,hidden,layers,ng", [
-    ("h2_burke", 256, 2, 1000), ("synth_gri53", 256, 4, 5000), ("h2_burke2012", 512, 2, 777),
+    # H = 256 picks its tile shape from the batch: at most 64 points per CU (4 wavefronts x 16
+    # points), 128 (8 x 16), more (8 x 32) -- ng 5000, 20000, 40000 on a 256-CU part
+    ("h2_burke", 256, 2, 1000), ("synth_gri53", 256, 4, 5000), ("synth_gri53", 256, 4, 20000),
+    ("synth_gri53", 256, 4, 40000), ("h2_burke2012", 512, 2, 777),
     ("synth_nhept88", 512, 3, 300),
     ("synth_jetsurf348", 512, 5, 600),   # the JetSurf-sized network: 352 -> 5 x 512 -> 348
 ])
