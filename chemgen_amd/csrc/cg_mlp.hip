@@ -955,12 +955,16 @@ int launch_infer(const double* P, const double* T, const double* Y, int64_t ld, 
   constexpr int kMaxLds = 160 * 1024;
   const int64_t lds = S::LDS_BYTES + (int64_t)ns * (8 + 4 + 4 * ne);
   if (lds > kMaxLds) return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: too many species for the LDS");
-  static bool attr_set = false;
-  if (!attr_set) {
+  // the LDS limit is raised once per device and kernel instance
+  static bool attr_set[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return cg_internal_fail(CG_EHIP, "cg_mlp_infer: no usable current device");
+  if (!attr_set[dev]) {
     hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_k<H, PTV, NWV>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
     if (e != hipSuccess) return cg_internal_fail(CG_EHIP, "cg_mlp_infer: cannot reserve LDS");
-    attr_set = true;
+    attr_set[dev] = true;
   }
   const int64_t nb = (ng + S::NW * S::NP - 1) / (S::NW * S::NP);
   if (nb > 0x7fffffffLL) return cg_internal_fail(CG_ERANGE, "cg_mlp_infer: ng too large");
@@ -987,14 +991,16 @@ extern "C" int cg_mlp_infer(const double* P, const double* T, const double* Y, i
     return cg_internal_fail(CG_EINVAL, "cg_mlp_infer: input/output wider than the hidden layers");
   if (hidden == 256) {
     // a batch of at most one 128-point workgroup per CU (the surrogate's audit, small calls)
-    // finishes in one workgroup's latency: half-size tiles halve it
-    static int n_cu = 0;
+    // finishes in one workgroup's latency: half-size tiles halve it.  CU count per device.
+    static int n_cu_dev[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+      return cg_internal_fail(CG_EHIP, "cg_mlp_infer: no usable current device");
+    int n_cu = n_cu_dev[dev];
     if (n_cu == 0) {
-      int dev = 0, v = 0;
-      if (hipGetDevice(&dev) != hipSuccess ||
-          hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
         return cg_internal_fail(CG_EHIP, "cg_mlp_infer: cannot read the CU count");
-      n_cu = v;
+      n_cu_dev[dev] = n_cu;
     }
     if (ng <= (int64_t)n_cu * 64)
       return launch_infer<256, 1, 4>(P, T, Y, ld, ng, ns, fscale, Wf, bias, kc0, n_hidden, nt_out, s, aek, ne, tol,
