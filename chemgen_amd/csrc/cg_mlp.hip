@@ -416,8 +416,9 @@ inline unsigned strided_grid(int64_t n) {
 
 // ---------------- fused inference ----------------
 //
-// One launch runs the whole network for 16*PT points per wavefront (4 wavefronts per
-// workgroup, one per SIMD) and decodes the result: features -> n_hidden SiLU layers of width
+// One launch runs the whole network for 16*PT points per wavefront (NW wavefronts per
+// workgroup: 8, two per SIMD, except for batches too small to fill the GPU) and decodes the
+// result: features -> n_hidden SiLU layers of width
 // H -> output layer -> wdot = s sinh(z) in FP64 -> element-conservation residual -> flag.
 // The activations never leave registers: the network is evaluated transposed,
 // out[n][m] = sum_k W[n][k] h[m][k], with the weights as the MFMA A operand (read from LDS)
